@@ -1,0 +1,107 @@
+/*
+ * todgpu.h -- C ABI of the MI355X (gfx950) beam-convolution TOD library
+ * (libtodgpu.so).
+ *
+ * All array arguments are DEVICE pointers (HIP global memory) unless stated
+ * otherwise; `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ * stream-ordered and asynchronous: the caller keeps every buffer alive until
+ * the stream has passed the call.  The library keeps no reference to caller
+ * buffers after a call returns.  Every entry point returns TODG_OK (0) or a
+ * negative TODG_ERR_* code; todg_last_error() gives the calling thread's last
+ * message.  No C++ exception crosses this boundary.
+ *
+ * Layouts (row-major, as in the reference):
+ *   vec_orig   (S, 3) float32  beam-frame unit vectors
+ *   beam_vals  (S,)   float32  beam weights
+ *   axes       (B, 3) float32  first Rodrigues axis       } the six arrays of
+ *   cos_a/sin_a (B,)  float32  first Rodrigues angle      } _rotation_params,
+ *   ax_pts     (B, 3) float32  boresight unit vector      } tod_rotations.py:306-333
+ *   cos_p/sin_p (B,)  float32  polarisation-roll angle    }
+ *   tod        (C, B) float64  accumulated in place (+=), as the reference
+ *   map_packed (npix, CS) float32, pixel-major, produced by todg_pack_map
+ *              from the reference's (C, npix) field-major mp_stacked.
+ */
+#ifndef TODGPU_H
+#define TODGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TODG_ABI_VERSION 1
+
+#define TODG_OK 0
+#define TODG_ERR_ARG -1 /* invalid argument (nside, C, c_q/c_u, null, range) */
+#define TODG_ERR_HIP -2 /* HIP runtime error (allocation, launch)          */
+
+/* ABI version of the loaded library (TODG_ABI_VERSION). */
+int todg_version(void);
+
+/* Copies the calling thread's last error message into buf (NUL-terminated,
+ * truncated to n-1 bytes); returns its length. */
+int todg_last_error(char *buf, size_t n);
+
+/* Packed-map layout for a beam entry with C channels and Q/U positions c_q,
+ * c_u (-1 = absent).  Writes the source channel of each packed slot (perm4,
+ * -1 = zero) and the tod row each slot accumulates into (row4); returns the
+ * slot stride CS (1, 2 or 4) or a negative error.  Replaces the channel
+ * bookkeeping of tod_core.py:105-112 and tod_bilinear.py:275-285. */
+int todg_map_slots(int C, int c_q, int c_u, int *perm4, int *row4);
+
+/* Converts the reference's field-major mp_stacked (C, npix) float32 (the
+ * array built at sample_based_tod_generation_gridint.py:447-450) into the
+ * pixel-major packed layout read by the gather kernels.  packed holds
+ * npix * CS floats (CS from todg_map_slots). */
+int todg_pack_map(const float *mp_stacked, int C, int64_t npix, int c_q, int c_u,
+                  float *packed, void *stream);
+
+/* Bilinear beam-convolution TOD with spin-2 Q/U correction.
+ * Replaces _gather_accum_fused_jit (tod_bilinear.py:205-222, same argument
+ * order; mp_stacked becomes map_packed + C).  tile = samples per wavefront
+ * (1, 2 or 4; anything else picks the default). */
+int todg_gather_accum_fused(const float *vec_orig, const float *axes, const float *cos_a,
+                            const float *sin_a, const float *ax_pts, const float *cos_p,
+                            const float *sin_p, int64_t nside, const float *map_packed, int C,
+                            const float *beam_vals, int64_t B, int64_t S, double *tod,
+                            int c_q, int c_u, int tile, void *stream);
+
+/* Nearest-pixel beam-convolution TOD.  Replaces _gather_accum_nearest_jit
+ * (tod_nearest.py:21-38, same argument order). */
+int todg_gather_accum_nearest(const float *vec_orig, const float *axes, const float *cos_a,
+                              const float *sin_a, const float *ax_pts, const float *cos_p,
+                              const float *sin_p, int64_t nside, const float *map_packed,
+                              int C, const float *beam_vals, int64_t B, int64_t S,
+                              double *tod, int c_q, int c_u, int tile, void *stream);
+
+/* Index-path dumps for samples [b0, b0+nb): the four bilinear neighbours and
+ * weights of every (b, s) as evaluated inside todg_gather_accum_fused
+ * (pix/wgt: (nb, S, 4) int64/float64), i.e. _rodrigues_apply_one_jit +
+ * _ring_interp_with_angles_jit (tod_rotations.py:83-134,
+ * numba_healpy.py:358-499). */
+int todg_bilinear_indices(const float *vec_orig, const float *axes, const float *cos_a,
+                          const float *sin_a, const float *ax_pts, const float *cos_p,
+                          const float *sin_p, int64_t nside, int64_t B, int64_t S,
+                          int64_t b0, int64_t nb, int64_t *pix, double *wgt, void *stream);
+
+/* The nearest pixel of every (b, s) as chosen inside todg_gather_accum_nearest
+ * (pix: (nb, S) int64), i.e. tod_nearest.py:91-141. */
+int todg_nearest_indices(const float *vec_orig, const float *axes, const float *cos_a,
+                         const float *sin_a, const float *ax_pts, const float *cos_p,
+                         const float *sin_p, int64_t nside, int64_t B, int64_t S,
+                         int64_t b0, int64_t nb, int64_t *pix, void *stream);
+
+/* HOST pointers: the per-ring constants the kernels use, rows 0..4*nside
+ * (ring index ir; rows 0 and 4*nside are zero padding).  Replaces
+ * _ring_info_jit / _ring_z_jit (numba_healpy.py:73-114); any output may be
+ * NULL. */
+int todg_ring_table(int64_t nside, double *z, double *theta, double *phi0, double *dphi,
+                    int64_t *first, int64_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TODGPU_H */

@@ -1,0 +1,112 @@
+"""ctypes binding of ``libtodgpu.so`` (declared in ``include/todgpu.h``).
+
+torch is imported first on purpose: the library's HIP runtime dependency
+(``libamdhip64.so.7``) then resolves to the runtime torch already loaded, so
+device pointers and streams from torch tensors are valid inside the library.
+There is no fallback: if the library is missing, every call raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  (must be loaded before the HIP library)
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libtodgpu.so")
+ABI_VERSION = 1
+
+TODG_OK = 0
+TODG_ERR_ARG = -1
+TODG_ERR_HIP = -2
+
+_lock = threading.Lock()
+_lib = None
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+
+# name -> (restype, argtypes); mirrors include/todgpu.h
+SIGNATURES = {
+    "todg_version": (_i32, []),
+    "todg_last_error": (_i32, [ctypes.c_char_p, ctypes.c_size_t]),
+    "todg_map_slots": (_i32, [_i32, _i32, _i32, _p, _p]),
+    "todg_pack_map": (_i32, [_p, _i32, _i64, _i32, _i32, _p, _p]),
+    "todg_gather_accum_fused": (
+        _i32,
+        [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _i32, _p],
+    ),
+    "todg_gather_accum_nearest": (
+        _i32,
+        [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _i32, _p],
+    ),
+    "todg_bilinear_indices": (
+        _i32,
+        [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p],
+    ),
+    "todg_nearest_indices": (
+        _i32,
+        [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i64, _p, _p],
+    ),
+    "todg_ring_table": (_i32, [_i64, _p, _p, _p, _p, _p, _p]),
+}
+
+
+class TodgError(RuntimeError):
+    """Error reported by libtodgpu (status code and message)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libtodgpu error {code}: {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """Load (once) and return the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with "
+                "`python -m tod_generation_mapbased_beam_amd.build` (or __graft_entry__.build())"
+            )
+        handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        v = handle.todg_version()
+        if v != ABI_VERSION:
+            raise RuntimeError(f"libtodgpu ABI {v} != expected {ABI_VERSION}")
+        _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    lib().todg_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int) -> int:
+    """Raise on a negative status: ValueError for argument errors (as the
+    reference raises for a bad interp_mode), TodgError for runtime errors."""
+    if rc < 0:
+        msg = last_error()
+        if rc == TODG_ERR_ARG:
+            raise ValueError(msg)
+        raise TodgError(rc, msg)
+    return rc
+
+
+def ptr(t) -> int:
+    """Device (or host) address of a tensor, 0 for None."""
+    if t is None:
+        return 0
+    return int(t.data_ptr())

@@ -1,0 +1,115 @@
+"""Device-memory plumbing: tensors in, C-ABI calls out.
+
+torch provides device memory and streams only; every computation runs in
+``libtodgpu.so``.  ``DeviceMap`` holds the pixel-major packed copy of a
+``(C, npix)`` float32 sky map in HBM (nside 2048, I/Q/U: 805 MB of the
+288 GB), built once and reused by every call, which is what the reference's
+POSIX shared-memory map (sample_based_tod_generation_gridint.py:478-507)
+provides on the CPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import weakref
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("tod_generation_mapbased_beam_amd needs a ROCm GPU (none visible)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def to_dev(x, dtype: torch.dtype) -> torch.Tensor:
+    """Contiguous device tensor of ``dtype`` (no copy when already so)."""
+    if isinstance(x, torch.Tensor):
+        t = x
+        if t.device.type != "cuda":
+            t = t.to(device())
+        if t.dtype != dtype:
+            t = t.to(dtype)
+        return t.contiguous()
+    arr = np.ascontiguousarray(np.asarray(x), dtype=torch.empty((), dtype=dtype).numpy().dtype)
+    return torch.from_numpy(arr).to(device())
+
+
+def map_slots(C: int, c_q: int, c_u: int):
+    perm = (ctypes.c_int * 4)()
+    row = (ctypes.c_int * 4)()
+    cs = _lib.check(_lib.lib().todg_map_slots(C, c_q, c_u, perm, row))
+    return cs, list(perm), list(row)
+
+
+def nside_of(npix: int) -> int:
+    nside = int(round((npix / 12) ** 0.5))
+    if 12 * nside * nside != npix or nside & (nside - 1):
+        raise ValueError(f"map length {npix} is not 12*nside^2 for a power-of-two nside")
+    return nside
+
+
+class DeviceMap:
+    """Packed device copy of a ``(C, npix)`` float32 map for a given Q/U
+    channel assignment (``c_q``, ``c_u``; -1 = absent)."""
+
+    def __init__(self, mp_stacked, c_q: int = -1, c_u: int = -1):
+        src = to_dev(mp_stacked, torch.float32)
+        if src.dim() == 1:
+            src = src[None, :]
+        if src.dim() != 2:
+            raise ValueError("mp_stacked must be (C, npix)")
+        self.C = int(src.shape[0])
+        self.npix = int(src.shape[1])
+        self.nside = nside_of(self.npix)
+        self.c_q = int(c_q)
+        self.c_u = int(c_u)
+        self.cs, self.perm, self.row = map_slots(self.C, self.c_q, self.c_u)
+        self.packed = torch.empty(self.npix * self.cs, dtype=torch.float32, device=src.device)
+        _lib.check(
+            _lib.lib().todg_pack_map(
+                _lib.ptr(src), self.C, self.npix, self.c_q, self.c_u, _lib.ptr(self.packed),
+                stream_handle(),
+            )
+        )
+
+    def matches(self, C: int, c_q: int, c_u: int) -> bool:
+        return (self.C, self.c_q, self.c_u) == (C, c_q, c_u)
+
+
+# Packed copies of host (numpy) maps, keyed by the array object (ndarrays
+# are unhashable, so by id() with a weak reference that drops the entry when
+# the array dies): the reference's production maps are read-only for the
+# life of the run.
+_host_cache: dict = {}
+
+
+def _host_entry(arr: np.ndarray) -> dict:
+    key = id(arr)
+    ent = _host_cache.get(key)
+    if ent is not None and ent[0]() is arr:
+        return ent[1]
+    per: dict = {}
+    _host_cache[key] = (weakref.ref(arr, lambda _r, k=key: _host_cache.pop(k, None)), per)
+    return per
+
+
+def device_map_for(mp_stacked, c_q: int, c_u: int) -> DeviceMap:
+    if isinstance(mp_stacked, DeviceMap):
+        if mp_stacked.c_q != c_q or mp_stacked.c_u != c_u:
+            raise ValueError("DeviceMap was packed for a different (c_q, c_u)")
+        return mp_stacked
+    if isinstance(mp_stacked, np.ndarray):
+        per = _host_entry(mp_stacked)
+        key = (c_q, c_u, mp_stacked.__array_interface__["data"][0], mp_stacked.shape)
+        if key not in per:
+            per[key] = DeviceMap(mp_stacked, c_q, c_u)
+        return per[key]
+    return DeviceMap(mp_stacked, c_q, c_u)

@@ -1,0 +1,106 @@
+"""Host-side rotation preparation (numpy), the counterpart of the reference's
+``precompute_rotation_vector_batch`` and ``_rotation_params``
+(tod_rotations.py:219-333).
+
+These run once per batch on the host in the reference too; their outputs (six
+float32 arrays per batch) are the per-sample operands of the GPU kernels.
+Parity tests feed the *same* arrays to the GPU and to the CPU oracle, so the
+bits of numpy's float32 cos/sin do not matter for kernel parity.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def _unit(phi, theta):
+    st = np.sin(theta)
+    return np.stack([st * np.cos(phi), st * np.sin(phi), np.cos(theta)], axis=-1)
+
+
+def precompute_rotation_vector_batch(ra, dec, phi_batch, theta_batch, center_idx=None):
+    """Rodrigues vectors taking the beam centre to each boresight, and the
+    parallactic offsets beta in [0, 2pi).  Same contract as
+    tod_rotations.py:219-303: ``ra``/``dec`` are (H, W) beam-frame offset
+    grids whose ``center_idx`` element (default ``(H//2, W//2)``) is the beam
+    centre; returns ``rot_vector (B, 3)`` float64 and ``beta (B,)`` float64.
+
+    A HEALPix beam centred on the north pole is ``ra=[[0.]], dec=[[pi/2]]``.
+    """
+    ra = np.asarray(ra, dtype=np.float64)
+    dec = np.asarray(dec, dtype=np.float64)
+    if center_idx is None:
+        center_idx = (ra.shape[0] // 2, ra.shape[1] // 2)
+    phi_c = ra[center_idx]
+    theta_c = np.pi / 2 - dec[center_idx]
+
+    phi_b = np.asarray(phi_batch)
+    theta_b = np.asarray(theta_batch)
+    c = _unit(phi_c, theta_c)[None, :]  # (1, 3) beam centre
+    t = _unit(phi_b, theta_b)  # (B, 3) boresight
+
+    # Local north at the beam centre and local north/east at the boresight.
+    n_c = np.array(
+        [np.cos(phi_c) * np.cos(theta_c), np.sin(phi_c) * np.cos(theta_c), -np.sin(theta_c)]
+    )
+    n_t = np.stack(
+        [np.cos(phi_b) * np.cos(theta_b), np.sin(phi_b) * np.cos(theta_b), -np.sin(theta_b)],
+        axis=-1,
+    )
+    e_t = np.stack([-np.sin(phi_b), np.cos(phi_b), np.zeros_like(phi_b)], axis=-1)
+
+    k = np.cross(c, t)
+    kn = np.linalg.norm(k, axis=-1, keepdims=True)
+    k = np.where(kn > 1e-10, k / np.where(kn > 0, kn, 1.0), 0.0)
+    ang = np.arccos(np.clip(np.sum(c * t, axis=-1), -1.0, 1.0))
+    rot_vector = k * ang[:, None]
+
+    # Rotate the centre's north with the same Rodrigues rotation, then read
+    # its bearing in the boresight's (north, east) frame.
+    ca = np.cos(ang)[:, None]
+    v = n_c[None, :]
+    w = v * ca + np.cross(k, v) * np.sin(ang)[:, None] + k * np.sum(k * v, axis=-1, keepdims=True) * (1.0 - ca)
+    beta = np.arctan2(np.sum(w * e_t, axis=-1), np.sum(w * n_t, axis=-1))
+    beta = np.where(beta < 0, beta + 2 * np.pi, beta)
+    return rot_vector, beta
+
+
+def _rotation_params(rot_vecs, phi_b, theta_b, psis_b):
+    """Per-sample float32 kernel operands, as tod_rotations.py:306-333:
+    ``axes (B,3), cos_a, sin_a (B,), ax_pts (B,3), cos_p, sin_p (B,)``.
+
+    The rotation angle is the float32 norm of the Rodrigues vector; the axis is
+    zero where that angle is <= 1e-10.  The boresight vector and the roll
+    angle's cos/sin are evaluated from float32 copies of the angles.
+    """
+    rot_vecs = np.asarray(rot_vecs)
+    ang = np.linalg.norm(rot_vecs, axis=-1).astype(np.float32)
+    ok = ang > np.float32(1e-10)
+    axes = (rot_vecs / np.where(ok[:, None], ang[:, None], np.float32(1.0))).astype(np.float32)
+    axes = np.where(ok[:, None], axes, np.float32(0.0))
+    cos_a = np.cos(ang)
+    sin_a = np.sin(ang)
+
+    ph = np.asarray(phi_b, dtype=np.float32)
+    th = np.asarray(theta_b, dtype=np.float32)
+    ps = np.asarray(psis_b, dtype=np.float32)
+    st = np.sin(th)
+    ax_pts = np.stack([st * np.cos(ph), st * np.sin(ph), np.cos(th)], axis=-1)
+    return axes, cos_a, sin_a, ax_pts, np.cos(ps), np.sin(ps)
+
+
+def rotation_params_for_pointing(theta, phi, psi, ra=None, dec=None, center_idx=None):
+    """Convenience: boresight (theta, phi, psi) -> the six kernel operands,
+    following the driver's casts (sample_based_tod_generation_gridint.py:
+    277-281): pointings are cast to float32 first, psi_b - beta combines the
+    roll.  Defaults to a HEALPix beam centred on the north pole."""
+    if ra is None:
+        ra = np.zeros((1, 1))
+        dec = np.full((1, 1), np.pi / 2)
+        center_idx = (0, 0)
+    th = np.asarray(theta).astype(np.float32)
+    ph = np.asarray(phi).astype(np.float32)
+    ps = np.asarray(psi).astype(np.float32)
+    rot, beta = precompute_rotation_vector_batch(ra, dec, ph, th, center_idx=center_idx)
+    psis = -beta + ps
+    return _rotation_params(rot, ph, th, psis)
