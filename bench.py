@@ -88,7 +88,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--tile", type=int, default=2)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="0 = footprint kernel, 1/2/4 = direct kernel with that sample tile")
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--nside-beam", type=int, default=4096)
     ap.add_argument("--fwhm", type=float, default=30.0)
@@ -127,7 +128,7 @@ def main():
         tod.zero_()
         if ev0 is not None:
             ev0.record()
-        fn(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, tile=args.tile)
+        fn(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, variant=args.variant)
         if ev1 is not None:
             ev1.record()
 
@@ -179,7 +180,7 @@ def main():
             "beam_pixels": S,
             "samples": B,
             "mode": args.mode,
-            "tile": args.tile,
+            "kernel_variant": args.variant,
             "map": "seeded Gaussian white noise (values do not affect the kernel's work)",
             "output_finite": finite,
         },

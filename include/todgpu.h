@@ -60,13 +60,16 @@ int todg_pack_map(const float *mp_stacked, int C, int64_t npix, int c_q, int c_u
 
 /* Bilinear beam-convolution TOD with spin-2 Q/U correction.
  * Replaces _gather_accum_fused_jit (tod_bilinear.py:205-222, same argument
- * order; mp_stacked becomes map_packed + C).  tile = samples per wavefront
- * (1, 2 or 4; anything else picks the default). */
+ * order; mp_stacked becomes map_packed + C).  variant selects the kernel:
+ * 0 = footprint kernel (per-sample LDS weight grid; the default), 8 = the
+ * footprint kernel compiled for 4 waves/SIMD, 1/2/4 = direct kernel with
+ * that many samples per wavefront.  Results agree to
+ * rounding; neighbour indices are identical. */
 int todg_gather_accum_fused(const float *vec_orig, const float *axes, const float *cos_a,
                             const float *sin_a, const float *ax_pts, const float *cos_p,
                             const float *sin_p, int64_t nside, const float *map_packed, int C,
                             const float *beam_vals, int64_t B, int64_t S, double *tod,
-                            int c_q, int c_u, int tile, void *stream);
+                            int c_q, int c_u, int variant, void *stream);
 
 /* Nearest-pixel beam-convolution TOD.  Replaces _gather_accum_nearest_jit
  * (tod_nearest.py:21-38, same argument order). */
@@ -74,7 +77,7 @@ int todg_gather_accum_nearest(const float *vec_orig, const float *axes, const fl
                               const float *sin_a, const float *ax_pts, const float *cos_p,
                               const float *sin_p, int64_t nside, const float *map_packed,
                               int C, const float *beam_vals, int64_t B, int64_t S,
-                              double *tod, int c_q, int c_u, int tile, void *stream);
+                              double *tod, int c_q, int c_u, int variant, void *stream);
 
 /* Index-path dumps for samples [b0, b0+nb): the four bilinear neighbours and
  * weights of every (b, s) as evaluated inside todg_gather_accum_fused

@@ -24,10 +24,10 @@ def K():
     return K
 
 
-def _gpu_tod(K, mode, vec, params, nside, mp, bv, B, S, c_q, c_u, tile=2):
+def _gpu_tod(K, mode, vec, params, nside, mp, bv, B, S, c_q, c_u, variant=0):
     tod = np.zeros((mp.shape[0], B))
     fn = K._gather_accum_fused_jit if mode == "bilinear" else K._gather_accum_nearest_jit
-    fn(vec, *params, nside, mp, bv, B, S, tod, c_q, c_u, tile=tile)
+    fn(vec, *params, nside, mp, bv, B, S, tod, c_q, c_u, variant=variant)
     return tod
 
 
@@ -38,12 +38,13 @@ def _cpu_tod(mode, vec, params, nside, mp, bv, B, S, c_q, c_u):
     return tod
 
 
+@pytest.mark.parametrize("variant", [0, 8, 2])
 @pytest.mark.parametrize("mode", ["bilinear", "nearest"])
 @pytest.mark.parametrize("qu", [(1, 2), (-1, -1)])
-def test_small_config_tod(K, mode, qu):
+def test_small_config_tod(K, mode, qu, variant):
     c = small_config()
     args = (c["vec"], c["params"], c["nside"], c["map"], c["bv"], c["B"], c["S"], *qu)
-    g = _gpu_tod(K, mode, *args)
+    g = _gpu_tod(K, mode, *args, variant=variant)
     r = _cpu_tod(mode, *args)
     err = rel_err(g, r)
     print(f"{mode} qu={qu} rel err {err:.3e}")
@@ -56,7 +57,7 @@ def test_small_config_bilinear_indices_bit_exact(K):
     pg, wg = K.bilinear_indices(c["vec"], *c["params"], c["nside"], c["B"], c["S"], 0, nb)
     pc, wc = O.fused_indices(c["vec"], *c["params"], c["nside"], 0, nb)
     assert np.array_equal(pg, pc), f"{np.count_nonzero(pg != pc)} index mismatches"
-    assert np.max(np.abs(wg - wc)) < 1e-12
+    assert np.max(np.abs(wg - wc)) < 1e-10  # weights are a tolerance path
 
 
 def test_small_config_nearest_indices_bit_exact(K):
@@ -76,35 +77,47 @@ def test_whole_sphere_indices_bit_exact(K, nside):
     pg, wg = K.bilinear_indices(vec, *params, nside, B, S, 0, B)
     pc, wc = O.fused_indices(vec, *params, nside, 0, B)
     assert np.array_equal(pg, pc)
-    assert np.max(np.abs(wg - wc)) < 1e-12
+    assert np.max(np.abs(wg - wc)) < 1e-10  # weights are a tolerance path
     ng = K.nearest_indices(vec, *params, nside, B, S, 0, B)
     nc = O.nearest_indices(vec, *params, nside, 0, B)
     assert np.array_equal(ng, nc)
 
 
+@pytest.mark.parametrize("variant", [0, 8, 1])
 @pytest.mark.parametrize("mode", ["bilinear", "nearest"])
 @pytest.mark.parametrize("nside", [4, 16, 64])
 @pytest.mark.parametrize("C,qu", [(3, (1, 2)), (3, (-1, -1)), (2, (0, 1)), (1, (-1, -1)), (2, (0, 0))])
-def test_random_rotations_tod(K, mode, nside, C, qu):
+def test_random_rotations_tod(K, mode, nside, C, qu, variant):
     rng = np.random.default_rng(7 * nside + C)
     B, S = 40, 300
     vec, params = random_rot_params(B, S, rng, pole_safe=False)
     mp = rng.uniform(-1.0, 1.0, (C, 12 * nside * nside)).astype(np.float32)
     bv = rng.uniform(0.1, 1.0, S).astype(np.float32)
     bv /= bv.sum()
-    g = _gpu_tod(K, mode, vec, params, nside, mp, bv, B, S, *qu)
+    g = _gpu_tod(K, mode, vec, params, nside, mp, bv, B, S, *qu, variant=variant)
     r = _cpu_tod(mode, vec, params, nside, mp, bv, B, S, *qu)
     assert rel_err(g, r) <= TOL
 
 
 @pytest.mark.parametrize("tile", [1, 2, 4])
-def test_tile_shapes_identical(K, tile):
+def test_direct_tile_shapes_identical(K, tile):
+    """The direct kernel's per-lane sums do not depend on its sample tile."""
     c = small_config()
     B = 3001
     params = tuple(p[:B] for p in c["params"])
-    ref = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2, tile=1)
-    got = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2, tile=tile)
+    ref = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2, variant=1)
+    got = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2, variant=tile)
     assert np.array_equal(ref, got)
+
+
+def test_footprint_deterministic(K):
+    """Same inputs, same footprint-kernel result, up to LDS summation order."""
+    c = small_config()
+    B = 3001
+    params = tuple(p[:B] for p in c["params"])
+    a = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2)
+    b = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2)
+    assert rel_err(a, b) <= 1e-12
 
 
 def test_accumulates_in_place(K):
@@ -115,7 +128,7 @@ def test_accumulates_in_place(K):
     K._gather_accum_fused_jit(c["vec"], *params, c["nside"], c["map"], c["bv"], B, c["S"], tod, 1, 2)
     first = tod.copy()
     K._gather_accum_fused_jit(c["vec"], *params, c["nside"], c["map"], c["bv"], B, c["S"], tod, 1, 2)
-    np.testing.assert_allclose(tod, 2 * first, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(tod, 2 * first, rtol=0, atol=1e-10)
 
 
 def test_zero_beam_and_constant_map(K):

@@ -6,7 +6,9 @@
 // it selects the same pixels as numba_healpy.py:53-114 and :358-499.  Per-ring
 // constants (z, acos(z), phi0, dphi, first pixel, pixel count) come from a
 // table built on the host with the same formulas and glibc's acos, so they
-// are bit-identical to what the reference computes inline.
+// are bit-identical to what the reference computes inline.  Everything that
+// only feeds weights or spin-2 factors (a tolerance path, not an index path)
+// uses explicit fma() and cheaper formulations.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,15 +19,19 @@ constexpr double kTwoPi = 2.0 * 3.141592653589793;  // numba_healpy.py:42
 constexpr double kPi = 3.141592653589793;
 constexpr double kTwoThirds = 2.0 / 3.0;            // numba_healpy.py:44
 
-// One row per ring index ir in [0, 4*nside]; rows 0 and 4*nside are unused
-// padding so that ir can index directly.  48 bytes, 16-byte aligned.
+// One row per ring index ir in [0, 4*nside]; rows 0 and 4*nside are padding
+// so that ir indexes directly.  80 bytes.
 struct alignas(16) RingEntry {
-    double z;       // ring centre cos(theta)         numba_healpy.py:104-114
-    double theta;   // acos(z), glibc                  numba_healpy.py:331-332
-    double phi0;    // longitude of first pixel        numba_healpy.py:99-100
-    double dphi;    // 2*pi / n                        numba_healpy.py:99
-    int64_t first;  // first pixel index               numba_healpy.py:85-98
-    int64_t n;      // pixels in the ring              numba_healpy.py:85-98
+    double z;        // ring centre cos(theta)                 numba_healpy.py:104-114
+    double theta;    // acos(z), glibc                          numba_healpy.py:331-332
+    double phi0;     // longitude of the first pixel            numba_healpy.py:99-100
+    double dphi;     // 2*pi / n                                numba_healpy.py:99
+    double inv_dphi; // 1 / dphi, correctly rounded             (exact slot division)
+    double sth;      // sqrt(max(0, 1 - z*z))                   tod_bilinear.py:194
+    double inv_dth;  // 1 / (theta[ir+1] - theta[ir])           (weights only)
+    int64_t first;   // first pixel index                       numba_healpy.py:85-98
+    int32_t n;       // pixels in the ring                      numba_healpy.py:85-98
+    int32_t pad;
 };
 
 // CPython float modulo (as lowered by Numba) for a positive modulus n and the
@@ -60,142 +66,116 @@ __device__ __forceinline__ int ring_above(int nside, double z) {
     return ir;
 }
 
-struct Bilin {
-    int64_t p[4];
-    double w[4];
-    double zn[4];
-    double phin[4];
+// x / d, correctly rounded, from the correctly rounded reciprocal inv = 1/d:
+// q0 = x*inv, residual r = x - q0*d exactly by FMA, q = q0 + r*inv.
+// Markstein's theorem makes q the correctly rounded quotient when nothing
+// underflows; tiny |x| takes the IEEE division.  (Also checked exhaustively
+// near every quotient boundary for all 8192 ring divisors 2*pi/n.)
+__device__ __forceinline__ double div_by(double x, double d, double inv) {
+    if (fabs(x) < 0x1p-960) return x / d;
+    const double q0 = x * inv;
+    const double r = fma(-q0, d, x);
+    return fma(r, inv, q0);
+}
+
+// Position of phi_w in ring r: tw = ((phi_w - phi0)/dphi) % n, i = int(tw),
+// frac = tw - i (numba_healpy.py:454-456).  i is in [0, n]; i == n only in
+// the reference's 1-ulp float-mod corner, which is reproduced.
+__device__ __forceinline__ int ring_slot(double phi_w, double phi0, double dphi,
+                                         double inv_dphi, int n, double& frac) {
+    const double tw = py_fmod_pos(div_by(phi_w - phi0, dphi, inv_dphi), (double)n);
+    const int i = (int)tw;
+    frac = tw - (double)i;
+    return i;
+}
+
+// (i + 1) % n for i in [0, n] without an integer division.
+__device__ __forceinline__ int next_slot(int i, int n) {
+    const int j = i + 1;
+    return j >= n ? j - n : j;
+}
+
+// ------------------------------------------------ tolerance-path helpers
+
+// sin and cos of |x| <= 0.3 by Taylor series (truncation < 1e-19).
+__device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
+    const double x2 = x * x;
+    double ps = fma(x2, 1.0 / 6227020800.0, -1.0 / 39916800.0);
+    ps = fma(x2, ps, 1.0 / 362880.0);
+    ps = fma(x2, ps, -1.0 / 5040.0);
+    ps = fma(x2, ps, 1.0 / 120.0);
+    ps = fma(x2, ps, -1.0 / 6.0);
+    s = fma(x * x2, ps, x);
+    double pc = fma(x2, 1.0 / 479001600.0, -1.0 / 3628800.0);
+    pc = fma(x2, pc, 1.0 / 40320.0);
+    pc = fma(x2, pc, -1.0 / 720.0);
+    pc = fma(x2, pc, 1.0 / 24.0);
+    pc = fma(x2, pc, -0.5);
+    c = fma(x2, pc, 1.0);
+}
+
+// sin and cos of the half angle x; only products of pairs are used
+// downstream, so (s, c) and (-s, -c) are interchangeable and x may be
+// reduced by pi.
+__device__ __forceinline__ void sincos_half(double x, double& s, double& c) {
+    if (fabs(x) <= 0.3) {
+        sincos_small(x, s, c);
+    } else {
+        sincos(x, &s, &c);
+    }
+}
+
+// 1/x from the hardware estimate plus two Newton steps (< 1 ulp error).
+__device__ __forceinline__ double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// Boresight-side and ring-side constants of the spin-2 formula
+// (tod_bilinear.py:111-137) that do not depend on the pixel's longitude.
+struct Spin2Ring {
+    double st2, S, s2dt, zsum, SC4;
 };
 
-// Bilinear neighbours, weights and neighbour (z, phi): numba_healpy.py:358-499.
-// theta_q = acos(z) only feeds the weights (tolerance path).
-__device__ __forceinline__ void ring_interp(int nside, const RingEntry* __restrict__ rt,
-                                            double z, double phi_w, Bilin& o) {
-    const int ir_above = ring_above(nside, z);
-    const int ir_below = ir_above + 1;
-    if (ir_above != 0 && ir_below != 4 * nside) {
-        const RingEntry ra = rt[ir_above];
-        const RingEntry rb = rt[ir_below];
-        const double theta = acos(z);
-        const double w_below = (theta - ra.theta) / (rb.theta - ra.theta);
-        const double w_above = 1.0 - w_below;
+__device__ __forceinline__ Spin2Ring spin2_ring(double z_pix, double sth_pix, double z_pts,
+                                                double sth_pts) {
+    Spin2Ring r;
+    const double dz = z_pts - z_pix;
+    const double ds = sth_pts - sth_pix;
+    r.st2 = 0.25 * fma(dz, dz, ds * ds);
+    r.S = sth_pix * sth_pts;
+    r.s2dt = 4.0 * r.st2 * (1.0 - r.st2);
+    r.zsum = z_pts + z_pix;
+    r.SC4 = 4.0 * r.S * (z_pts * z_pix);
+    return r;
+}
 
-        double tw = py_fmod_pos((phi_w - ra.phi0) / ra.dphi, (double)ra.n);
-        int64_t ia = (int64_t)tw;
-        double fa = tw - (double)ia;
-        int64_t ia1 = (ia + 1) % ra.n;
-        o.p[0] = ra.first + ia;
-        o.p[1] = ra.first + ia1;
-        o.w[0] = w_above * (1.0 - fa);
-        o.w[1] = w_above * fa;
-
-        tw = py_fmod_pos((phi_w - rb.phi0) / rb.dphi, (double)rb.n);
-        int64_t ib = (int64_t)tw;
-        double fb = tw - (double)ib;
-        int64_t ib1 = (ib + 1) % rb.n;
-        o.p[2] = rb.first + ib;
-        o.p[3] = rb.first + ib1;
-        o.w[2] = w_below * (1.0 - fb);
-        o.w[3] = w_below * fb;
-
-        o.zn[0] = ra.z;
-        o.zn[1] = ra.z;
-        o.zn[2] = rb.z;
-        o.zn[3] = rb.z;
-        o.phin[0] = ra.phi0 + (double)ia * ra.dphi;
-        o.phin[1] = ra.phi0 + (double)ia1 * ra.dphi;
-        o.phin[2] = rb.phi0 + (double)ib * rb.dphi;
-        o.phin[3] = rb.phi0 + (double)ib1 * rb.dphi;
+// cos(2 delta), sin(2 delta) given (sin, cos) of half the longitude gap
+// (tod_bilinear.py:112-145).  The reference's exact-coincidence guard
+// returns (1, 0); for coincident points N = D = 0, which is tested here.
+__device__ __forceinline__ void spin2_from_half(const Spin2Ring& r, double sh, double ch,
+                                                double& c2d, double& s2d) {
+    const double sin_dphi = 2.0 * ch * sh;
+    const double sh2 = sh * sh;
+    const double cos_dphi = fma(-2.0, sh2, 1.0);
+    const double h = fma(r.S, sh2, r.st2);
+    const double N = 2.0 * sin_dphi * r.zsum * h;
+    const double D = fma(r.S * sin_dphi, sin_dphi, fma(r.s2dt, cos_dphi, -(r.SC4 * sh2 * sh2)));
+    const double den = fma(N, N, D * D);
+    if (den == 0.0) {
+        c2d = 1.0;
+        s2d = 0.0;
         return;
     }
-    const double theta = acos(z);
-    if (ir_above == 0) {
-        // North-pole boundary: numba_healpy.py:384-411
-        const RingEntry r1 = rt[1];
-        double tw = py_fmod_pos((phi_w - r1.phi0) / r1.dphi, (double)r1.n);
-        int64_t ip = (int64_t)tw;
-        double frac = tw - (double)ip;
-        int64_t ip2 = (ip + 1) % r1.n;
-        int64_t q0 = (ip + 2) % r1.n, q1 = (ip2 + 2) % r1.n;
-        o.p[0] = r1.first + q0;
-        o.p[1] = r1.first + q1;
-        o.p[2] = r1.first + ip;
-        o.p[3] = r1.first + ip2;
-        double w_theta = theta / r1.theta;
-        double nf = (1.0 - w_theta) * 0.25;
-        o.w[0] = nf;
-        o.w[1] = nf;
-        o.w[2] = (1.0 - frac) * w_theta + nf;
-        o.w[3] = frac * w_theta + nf;
-        o.zn[0] = o.zn[1] = o.zn[2] = o.zn[3] = r1.z;
-        o.phin[0] = r1.phi0 + (double)q0 * r1.dphi;
-        o.phin[1] = r1.phi0 + (double)q1 * r1.dphi;
-        o.phin[2] = r1.phi0 + (double)ip * r1.dphi;
-        o.phin[3] = r1.phi0 + (double)ip2 * r1.dphi;
-    } else {
-        // South-pole boundary: numba_healpy.py:413-441
-        const RingEntry rl = rt[4 * nside - 1];
-        double tw = py_fmod_pos((phi_w - rl.phi0) / rl.dphi, (double)rl.n);
-        int64_t ip = (int64_t)tw;
-        double frac = tw - (double)ip;
-        int64_t ip2 = (ip + 1) % rl.n;
-        int64_t q2 = (ip + 2) % rl.n, q3 = (ip2 + 2) % rl.n;
-        o.p[0] = rl.first + ip;
-        o.p[1] = rl.first + ip2;
-        o.p[2] = rl.first + q2;
-        o.p[3] = rl.first + q3;
-        double wts = (theta - rl.theta) / (kPi - rl.theta);
-        double sf = wts * 0.25;
-        o.w[0] = (1.0 - frac) * (1.0 - wts) + sf;
-        o.w[1] = frac * (1.0 - wts) + sf;
-        o.w[2] = sf;
-        o.w[3] = sf;
-        o.zn[0] = o.zn[1] = o.zn[2] = o.zn[3] = rl.z;
-        o.phin[0] = rl.phi0 + (double)ip * rl.dphi;
-        o.phin[1] = rl.phi0 + (double)ip2 * rl.dphi;
-        o.phin[2] = rl.phi0 + (double)q2 * rl.dphi;
-        o.phin[3] = rl.phi0 + (double)q3 * rl.dphi;
-    }
+    const double u = rcp_nr(den);
+    c2d = fma(D, D, -(N * N)) * u;
+    s2d = 2.0 * N * D * u;
 }
 
-// The nearest kernel's 4-candidate search: tod_nearest.py:112-141.
-// z is not clamped here (tod_nearest.py:112); sin_th comes from the caller.
-__device__ __forceinline__ void nearest_pix(int nside, const RingEntry* __restrict__ rt,
-                                            double z, double sin_th, double phi_w,
-                                            int64_t& best_pix, double& best_z,
-                                            double& best_phi) {
-    int ir_above = ring_above(nside, z);
-    if (ir_above < 1)
-        ir_above = 1;
-    else if (ir_above > 4 * nside - 2)
-        ir_above = 4 * nside - 2;
-    double bcos = -2.0;
-    best_pix = 0;
-    best_z = 0.0;
-    best_phi = 0.0;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int ir = ir_above + k;
-        const RingEntry r = rt[ir];
-        const double sin_z_c = sqrt(fmax(0.0, 1.0 - r.z * r.z));
-        const int64_t ip_base = ((int64_t)(phi_w * (double)r.n / kTwoPi)) % r.n;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t ip_try = j == 0 ? ip_base : (ip_base + 1) % r.n;
-            const double phi_c = r.phi0 + (double)ip_try * r.dphi;
-            const double cos_d = sin_th * sin_z_c * cos(phi_w - phi_c) + z * r.z;
-            if (cos_d > bcos) {
-                bcos = cos_d;
-                best_pix = r.first + ip_try;
-                best_z = r.z;
-                best_phi = phi_c;
-            }
-        }
-    }
-}
-
-// cos(2 delta), sin(2 delta) of the spin-2 frame rotation between a map pixel
-// and the boresight: tod_bilinear.py:84-145, same formula and order.
+// Full spin-2 for one pixel (generic longitude gap).
 __device__ __forceinline__ void spin2(double z_pix, double sth_pix, double phi_pix,
                                       double z_pts, double sth_pts, double phi_pts,
                                       double& c2d, double& s2d) {
@@ -204,38 +184,35 @@ __device__ __forceinline__ void spin2(double z_pix, double sth_pix, double phi_p
         s2d = 0.0;
         return;
     }
-    const double dphi = phi_pts - phi_pix;
     double sh, ch;
-    sincos(dphi * 0.5, &sh, &ch);
-    const double sin_dphi = 2.0 * ch * sh;
-    const double cos_dphi = 1.0 - 2.0 * sh * sh;
-    const double sh2 = sh * sh;
-    const double dz = z_pts - z_pix;
-    const double ds = sth_pts - sth_pix;
-    const double st2 = 0.25 * (dz * dz + ds * ds);
-    const double S = sth_pix * sth_pts;
-    const double h = st2 + S * sh2;
-    const double sin2_dtheta = 4.0 * st2 * (1.0 - st2);
-    const double z_sum = z_pts + z_pix;
-    const double N = 2.0 * sin_dphi * z_sum * h;
-    const double C = z_pts * z_pix;
-    const double term1 = sin2_dtheta * cos_dphi;
-    const double term2 = 4.0 * S * C * sh2 * sh2;
-    const double term3 = S * sin_dphi * sin_dphi;
-    const double D = term1 - term2 + term3;
-    const double u = 1.0 / (N * N + D * D);
-    c2d = (D * D - N * N) * u;
-    s2d = 2.0 * N * D * u;
+    sincos_half((phi_pts - phi_pix) * 0.5, sh, ch);
+    spin2_from_half(spin2_ring(z_pix, sth_pix, z_pts, sth_pts), sh, ch, c2d, s2d);
+}
+
+// theta - theta_a for a query z below ring a (z <= za): asin(sin(theta -
+// theta_a)) with sin(theta - theta_a) = sqrt(1-z^2) za - z sqrt(1-za^2),
+// by its series when small, else via acos.  Weights only.
+__device__ __forceinline__ double dtheta_from(double z, double za, double sa, double theta_a) {
+    const double sq = sqrt(fmax(0.0, (1.0 - z) * (1.0 + z)));
+    const double x = fma(sq, za, -(z * sa));
+    if (fabs(x) < 1e-2) {
+        const double x2 = x * x;
+        // asin(x) = x + x^3/6 + 3x^5/40 + 15x^7/336 + 105x^9/3456
+        double p = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
+        p = fma(x2, p, 3.0 / 40.0);
+        p = fma(x2, p, 1.0 / 6.0);
+        return fma(x * x2, p, x);
+    }
+    return acos(z) - theta_a;
 }
 
 // Double Rodrigues rotation, tod_rotations.py:122-133, binary64, same order.
+// oma = 1 - ca and omp = 1 - cp_ are computed by the caller once per sample.
 __device__ __forceinline__ void rodrigues(double vx_in, double vy_in, double vz_in,
                                           double kx, double ky, double kz, double ca,
-                                          double sa, double px, double py, double pz,
-                                          double cp_, double sp_, double& vx, double& vy,
-                                          double& vz) {
-    const double oma = 1.0 - ca;
-    const double omp = 1.0 - cp_;
+                                          double sa, double oma, double px, double py,
+                                          double pz, double cp_, double sp_, double omp,
+                                          double& vx, double& vy, double& vz) {
     const double dkv = kx * vx_in + ky * vy_in + kz * vz_in;
     const double rx = vx_in * ca + (ky * vz_in - kz * vy_in) * sa + kx * dkv * oma;
     const double ry = vy_in * ca + (kz * vx_in - kx * vz_in) * sa + ky * dkv * oma;
@@ -244,6 +221,115 @@ __device__ __forceinline__ void rodrigues(double vx_in, double vy_in, double vz_
     vx = rx * cp_ + (py * rz - pz * ry) * sp_ + px * dpr * omp;
     vy = ry * cp_ + (pz * rx - px * rz) * sp_ + py * dpr * omp;
     vz = rz * cp_ + (px * ry - py * rx) * sp_ + pz * dpr * omp;
+}
+
+// One bilinear neighbour: ring index, slot in the ring, global pixel, weight
+// and the pixel centre (z, phi) used for spin-2.
+struct Nb {
+    int ring[4];
+    int pos[4];
+    double w[4];
+};
+
+// Bilinear neighbours (numba_healpy.py:358-499) from the global ring table:
+// rings, slots and weights.  Pixel index = rt[ring].first + pos; pixel
+// centre = (rt[ring].z, rt[ring].phi0 + pos * rt[ring].dphi).
+__device__ __forceinline__ void ring_interp(int nside, const RingEntry* __restrict__ rt,
+                                            double z, double phi_w, Nb& o) {
+    const int ir_above = ring_above(nside, z);
+    const int ir_below = ir_above + 1;
+    if (ir_above != 0 && ir_below != 4 * nside) {
+        const RingEntry& ra = rt[ir_above];
+        const RingEntry& rb = rt[ir_below];
+        const double dth = dtheta_from(z, ra.z, ra.sth, ra.theta);
+        const double w_below = dth * ra.inv_dth;
+        const double w_above = 1.0 - w_below;
+        double fa, fb;
+        const int ia = ring_slot(phi_w, ra.phi0, ra.dphi, ra.inv_dphi, ra.n, fa);
+        const int ib = ring_slot(phi_w, rb.phi0, rb.dphi, rb.inv_dphi, rb.n, fb);
+        o.ring[0] = o.ring[1] = ir_above;
+        o.ring[2] = o.ring[3] = ir_below;
+        o.pos[0] = ia;
+        o.pos[1] = next_slot(ia, ra.n);
+        o.pos[2] = ib;
+        o.pos[3] = next_slot(ib, rb.n);
+        o.w[0] = w_above * (1.0 - fa);
+        o.w[1] = w_above * fa;
+        o.w[2] = w_below * (1.0 - fb);
+        o.w[3] = w_below * fb;
+        return;
+    }
+    const double theta = acos(z);
+    if (ir_above == 0) {
+        // North-pole boundary: numba_healpy.py:384-411
+        const RingEntry& r1 = rt[1];
+        double frac;
+        const int ip = ring_slot(phi_w, r1.phi0, r1.dphi, r1.inv_dphi, r1.n, frac);
+        const int ip2 = next_slot(ip, r1.n);
+        o.ring[0] = o.ring[1] = o.ring[2] = o.ring[3] = 1;
+        o.pos[0] = (ip + 2) % r1.n;
+        o.pos[1] = (ip2 + 2) % r1.n;
+        o.pos[2] = ip;
+        o.pos[3] = ip2;
+        const double w_theta = theta / r1.theta;
+        const double nf = (1.0 - w_theta) * 0.25;
+        o.w[0] = nf;
+        o.w[1] = nf;
+        o.w[2] = (1.0 - frac) * w_theta + nf;
+        o.w[3] = frac * w_theta + nf;
+    } else {
+        // South-pole boundary: numba_healpy.py:413-441
+        const int irl = 4 * nside - 1;
+        const RingEntry& rl = rt[irl];
+        double frac;
+        const int ip = ring_slot(phi_w, rl.phi0, rl.dphi, rl.inv_dphi, rl.n, frac);
+        const int ip2 = next_slot(ip, rl.n);
+        o.ring[0] = o.ring[1] = o.ring[2] = o.ring[3] = irl;
+        o.pos[0] = ip;
+        o.pos[1] = ip2;
+        o.pos[2] = (ip + 2) % rl.n;
+        o.pos[3] = (ip2 + 2) % rl.n;
+        const double wts = (theta - rl.theta) / (kPi - rl.theta);
+        const double sf = wts * 0.25;
+        o.w[0] = (1.0 - frac) * (1.0 - wts) + sf;
+        o.w[1] = frac * (1.0 - wts) + sf;
+        o.w[2] = sf;
+        o.w[3] = sf;
+    }
+}
+
+// The nearest kernel's 4-candidate search: tod_nearest.py:112-141.
+// z is not clamped here (tod_nearest.py:112); sin_th comes from the caller.
+__device__ __forceinline__ void nearest_pix(int nside, const RingEntry* __restrict__ rt,
+                                            double z, double sin_th, double phi_w,
+                                            int& best_ring, int& best_pos) {
+    int ir_above = ring_above(nside, z);
+    if (ir_above < 1)
+        ir_above = 1;
+    else if (ir_above > 4 * nside - 2)
+        ir_above = 4 * nside - 2;
+    double bcos = -2.0;
+    best_ring = ir_above;
+    best_pos = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int ir = ir_above + k;
+        const RingEntry& r = rt[ir];
+        const double sin_z_c = sqrt(fmax(0.0, 1.0 - r.z * r.z));
+        int ip_base = (int)(phi_w * (double)r.n / kTwoPi);
+        if (ip_base >= r.n) ip_base -= r.n;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int ip_try = j == 0 ? ip_base : next_slot(ip_base, r.n);
+            const double phi_c = r.phi0 + (double)ip_try * r.dphi;
+            const double cos_d = sin_th * sin_z_c * cos(phi_w - phi_c) + z * r.z;
+            if (cos_d > bcos) {
+                bcos = cos_d;
+                best_ring = ir;
+                best_pos = ip_try;
+            }
+        }
+    }
 }
 
 }  // namespace todg

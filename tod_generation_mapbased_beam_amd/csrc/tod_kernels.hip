@@ -2,15 +2,30 @@
 //
 //   tod[c, b] += sum_s beam_vals[s] * spin2_c( map[ R_b * vec_orig[s] ] )
 //
-// Work decomposition: one wavefront owns a tile of T consecutive samples and
-// sweeps the beam with its 64 lanes (lane l handles beam pixels l, l+64, ...).
-// The per-sample rotation scalars are wave-uniform; the rotated direction,
-// the HEALPix ring lookup (binary64, reference operation order) and the
-// spin-2 factors are computed per lane; the I/Q/U values of each neighbour
-// come from one 16-byte load of a pixel-major packed map.  Each lane keeps
-// binary64 partial sums (binary64 add issues at the full VALU rate on CDNA4)
-// and the wave reduces them across lanes once per sample, so each sample's
-// tod entry is written by exactly one lane: no atomics, deterministic.
+// Two kernels share one device index path (binary64, reference operation
+// order, see healpix_ring.hpp):
+//
+// * k_tod_fp (default): one workgroup per sample at a time.  The sample's
+//   footprint on the map -- the rings and ring slots that the rotated beam
+//   and its bilinear neighbours can touch -- is laid out in LDS.  Phase 1
+//   sweeps the beam: for every beam pixel it evaluates the rotated direction
+//   and its neighbours and adds beam_val * weight into a per-pixel LDS
+//   weight grid (ds_add_f64).  Phase 2 sweeps the footprint once: per map
+//   pixel it computes the spin-2 factors and gathers I/Q/U with one 16-byte
+//   load.  Spin-2 and the map gather therefore run once per footprint pixel
+//   (~2.3k per sample at nside 2048 / 30') instead of once per neighbour
+//   (4 x S ~ 25k), which is the per-sample spin-2 memo of the reference
+//   (tod_bilinear.py:151-199) taken to its limit.  A neighbour that falls
+//   outside the laid-out footprint is accumulated directly (spin-2 + gather
+//   on the spot), so the result never depends on the footprint estimate.
+//
+// * k_tod_direct: one wavefront owns T consecutive samples and sweeps the
+//   beam with its lanes, evaluating spin-2 and gathering per neighbour.
+//   Kept as a cross-check and for callers that pick it explicitly.
+//
+// Both keep binary64 sums per lane (binary64 adds issue at the full VALU
+// rate on CDNA4) and reduce across lanes once per sample; each sample's tod
+// entry is written by exactly one thread, so there are no atomics on tod.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -46,7 +61,8 @@ int fail(int code, const std::string& msg) {
 // ---------------------------------------------------------------- ring table
 
 // Host construction of the per-ring constants with the reference formulas
-// (numba_healpy.py:73-114) and glibc acos (numba_healpy.py:331-332).
+// (numba_healpy.py:73-114), glibc acos (numba_healpy.py:331-332) and sqrt
+// (tod_bilinear.py:194).  volatile keeps each rounding step in source order.
 std::vector<RingEntry> build_ring_table(int64_t nside) {
     const double two_pi = 2.0 * 3.141592653589793;
     const int64_t npix = 12 * nside * nside;
@@ -74,22 +90,31 @@ std::vector<RingEntry> build_ring_table(int64_t nside) {
         if (ir < nside) {
             volatile double tmp = (double)ir;
             volatile double den = 3.0 * (double)nside * (double)nside;
-            z = 1.0 - tmp * tmp / den;
+            volatile double q = tmp * tmp / den;
+            z = 1.0 - q;
         } else if (ir <= 3 * nside) {
             volatile double q = (double)ir / (double)nside;
             z = (2.0 / 3.0) * (2.0 - q);
         } else {
             volatile double tmp = (double)(4 * nside - ir);
             volatile double den = 3.0 * (double)nside * (double)nside;
-            z = -(1.0 - tmp * tmp / den);
+            volatile double q = tmp * tmp / den;
+            z = -(1.0 - q);
         }
         RingEntry& e = t[(size_t)ir];
         e.z = z;
         e.theta = acos(z);
         e.phi0 = phi0;
         e.dphi = dphi;
+        e.inv_dphi = 1.0 / dphi;
+        volatile double zz = z * z;
+        e.sth = sqrt(1.0 - zz > 0.0 ? 1.0 - zz : 0.0);
         e.first = fp;
-        e.n = n;
+        e.n = (int32_t)n;
+    }
+    for (int64_t ir = 1; ir < 4 * nside - 1; ++ir) {
+        volatile double d = t[(size_t)ir + 1].theta - t[(size_t)ir].theta;
+        t[(size_t)ir].inv_dth = 1.0 / d;
     }
     return t;
 }
@@ -159,7 +184,7 @@ struct TodArgs {
 };
 
 struct Sample {
-    double kx, ky, kz, ca, sa, bx, by, bz, cp, sp;
+    double kx, ky, kz, ca, sa, oma, bx, by, bz, cp, sp, omp;
     double z_pts, sth_pts, phi_pts;
 };
 
@@ -169,17 +194,57 @@ __device__ __forceinline__ void load_sample(const TodArgs& a, int64_t b, Sample&
     P.kz = (double)a.axes[3 * b + 2];
     P.ca = (double)a.cos_a[b];
     P.sa = (double)a.sin_a[b];
+    P.oma = 1.0 - P.ca;
     P.bx = (double)a.ax_pts[3 * b];
     P.by = (double)a.ax_pts[3 * b + 1];
     P.bz = (double)a.ax_pts[3 * b + 2];
     P.cp = (double)a.cos_p[b];
     P.sp = (double)a.sin_p[b];
+    P.omp = 1.0 - P.cp;
     // Boresight (z, sin theta, phi) for spin-2: tod_bilinear.py:306-310.
     P.z_pts = fmax(-1.0, fmin(1.0, P.bz));
     P.sth_pts = sqrt(fmax(0.0, 1.0 - P.bz * P.bz));
     double ph = atan2(P.by, P.bx);
     if (ph < 0.0) ph += kTwoPi;
     P.phi_pts = ph;
+}
+
+// Broadcast lane 0's value to the wave (an SGPR value for the compiler).
+__device__ __forceinline__ double uniform(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ void make_uniform(Sample& P) {
+    P.kx = uniform(P.kx);
+    P.ky = uniform(P.ky);
+    P.kz = uniform(P.kz);
+    P.ca = uniform(P.ca);
+    P.sa = uniform(P.sa);
+    P.oma = uniform(P.oma);
+    P.bx = uniform(P.bx);
+    P.by = uniform(P.by);
+    P.bz = uniform(P.bz);
+    P.cp = uniform(P.cp);
+    P.sp = uniform(P.sp);
+    P.omp = uniform(P.omp);
+    P.z_pts = uniform(P.z_pts);
+    P.sth_pts = uniform(P.sth_pts);
+    P.phi_pts = uniform(P.phi_pts);
+}
+
+__device__ __forceinline__ void rotate(const Sample& q, double v0, double v1, double v2,
+                                       double& vx, double& vy, double& vz) {
+    rodrigues(v0, v1, v2, q.kx, q.ky, q.kz, q.ca, q.sa, q.oma, q.bx, q.by, q.bz, q.cp, q.sp,
+              q.omp, vx, vy, vz);
+}
+
+__device__ __forceinline__ double wrap_phi(double y, double x) {
+    double p = atan2(y, x);
+    if (p < 0.0) p += kTwoPi;
+    return p;
 }
 
 template <int CS>
@@ -190,7 +255,7 @@ __device__ __forceinline__ float4 load_px(const float* __restrict__ map, int64_t
     // catches negatives) so such cases read a valid pixel, never fault.
     p = (uint64_t)p < (uint64_t)npix ? p : ((p < 0) ? 0 : npix - 1);
     if constexpr (CS == 4) {
-        return *reinterpret_cast<const float4*>(map + 4 * p);
+        return *(reinterpret_cast<const float4*>(map) + p);
     } else if constexpr (CS == 2) {
         float2 v = *reinterpret_cast<const float2*>(map + 2 * p);
         return make_float4(v.x, v.y, 0.f, 0.f);
@@ -209,18 +274,47 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// Accumulate w * map-pixel (with spin-2 rotation of slots 0/1 when QU) into
+// acc.  Pixel centre (z_n, sth_n, phi_n) feeds spin-2.
+template <bool QU, int CS>
+__device__ __forceinline__ void accum_pixel(const TodArgs& a, const Sample& q, int64_t p,
+                                            double z_n, double sth_n, double phi_n, double w,
+                                            double* acc) {
+    const float4 m = load_px<CS>(a.map, p, a.npix);
+    if constexpr (QU) {
+        double c2d, s2d;
+        spin2(z_n, sth_n, phi_n, q.z_pts, q.sth_pts, q.phi_pts, c2d, s2d);
+        const double qv = (double)m.x, uv = (double)m.y;
+        acc[0] = fma(w, fma(qv, c2d, uv * s2d), acc[0]);
+        acc[1] = fma(w, fma(uv, c2d, -(qv * s2d)), acc[1]);
+#pragma unroll
+        for (int k = 2; k < CS; ++k)
+            if (k < a.nslots) acc[k] = fma(w, (double)slot(m, k), acc[k]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < CS; ++k)
+            if (k < a.nslots) acc[k] = fma(w, (double)slot(m, k), acc[k]);
+    }
+}
+
+template <bool QU, int CS>
+__device__ __forceinline__ void accum_ring_pos(const TodArgs& a, const Sample& q, int ring,
+                                               int pos, double w, double* acc) {
+    const RingEntry& e = a.rings[ring];
+    accum_pixel<QU, CS>(a, q, e.first + pos, e.z, e.sth, e.phi0 + (double)pos * e.dphi, w, acc);
+}
+
+// ============================================================ direct kernel
+
 // MODE 0 = bilinear (tod_bilinear.py:205-477), 1 = nearest (tod_nearest.py:21-158).
-// QU: slots 0 and 1 hold Q and U and get the spin-2 rotation; other slots
-// are accumulated plainly.  Without QU every slot is accumulated plainly.
 template <int MODE, bool QU, int CS, int T>
-__global__ __launch_bounds__(256) void k_tod(const TodArgs a) {
+__global__ __launch_bounds__(256) void k_tod_direct(const TodArgs a) {
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t wpb = blockDim.x >> 6;
     const int64_t wave0 = (int64_t)blockIdx.x * wpb + wib;
     const int64_t nwaves = (int64_t)gridDim.x * wpb;
     const int64_t ntiles = (a.B + T - 1) / T;
-    const int nslots = a.nslots;
 
     for (int64_t tile = wave0; tile < ntiles; tile += nwaves) {
         const int64_t b0 = tile * T;
@@ -233,7 +327,6 @@ __global__ __launch_bounds__(256) void k_tod(const TodArgs a) {
 #pragma unroll
             for (int k = 0; k < kMaxSlots; ++k) acc[t][k] = 0.0;
         }
-
         for (int64_t s = lane; s < a.S; s += 64) {
             const double v0 = (double)a.vec_orig[3 * s];
             const double v1 = (double)a.vec_orig[3 * s + 1];
@@ -241,94 +334,29 @@ __global__ __launch_bounds__(256) void k_tod(const TodArgs a) {
             const double bv = (double)a.beam_vals[s];
 #pragma unroll
             for (int t = 0; t < T; ++t) {
-                const Sample& q = P[t];
                 double vx, vy, vz;
-                rodrigues(v0, v1, v2, q.kx, q.ky, q.kz, q.ca, q.sa, q.bx, q.by, q.bz, q.cp,
-                          q.sp, vx, vy, vz);
-                double phi_w = atan2(vy, vx);
-                if (phi_w < 0.0) phi_w += kTwoPi;
+                rotate(P[t], v0, v1, v2, vx, vy, vz);
+                const double phi_w = wrap_phi(vy, vx);
                 if constexpr (MODE == 0) {
-                    const double z = fmax(-1.0, fmin(1.0, vz));
-                    Bilin o;
-                    ring_interp(a.nside, a.rings, z, phi_w, o);
-                    float4 m[4];
+                    Nb o;
+                    ring_interp(a.nside, a.rings, fmax(-1.0, fmin(1.0, vz)), phi_w, o);
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) m[k] = load_px<CS>(a.map, o.p[k], a.npix);
-                    if constexpr (QU) {
-                        double c2[4], s2[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const double sth_n = sqrt(fmax(0.0, 1.0 - o.zn[k] * o.zn[k]));
-                            spin2(o.zn[k], sth_n, o.phin[k], q.z_pts, q.sth_pts, q.phi_pts,
-                                  c2[k], s2[k]);
-                        }
-                        double qq[4], uu[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            qq[k] = (double)m[k].x;
-                            uu[k] = (double)m[k].y;
-                        }
-                        acc[t][0] += (o.w[0] * (qq[0] * c2[0] + uu[0] * s2[0]) +
-                                      o.w[1] * (qq[1] * c2[1] + uu[1] * s2[1]) +
-                                      o.w[2] * (qq[2] * c2[2] + uu[2] * s2[2]) +
-                                      o.w[3] * (qq[3] * c2[3] + uu[3] * s2[3])) *
-                                     bv;
-                        acc[t][1] += (o.w[0] * (-qq[0] * s2[0] + uu[0] * c2[0]) +
-                                      o.w[1] * (-qq[1] * s2[1] + uu[1] * c2[1]) +
-                                      o.w[2] * (-qq[2] * s2[2] + uu[2] * c2[2]) +
-                                      o.w[3] * (-qq[3] * s2[3] + uu[3] * c2[3])) *
-                                     bv;
-#pragma unroll
-                        for (int k = 2; k < CS; ++k) {
-                            if (k < nslots)
-                                acc[t][k] += (o.w[0] * (double)slot(m[0], k) +
-                                              o.w[1] * (double)slot(m[1], k) +
-                                              o.w[2] * (double)slot(m[2], k) +
-                                              o.w[3] * (double)slot(m[3], k)) *
-                                             bv;
-                        }
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < CS; ++k) {
-                            if (k < nslots)
-                                acc[t][k] += (o.w[0] * (double)slot(m[0], k) +
-                                              o.w[1] * (double)slot(m[1], k) +
-                                              o.w[2] * (double)slot(m[2], k) +
-                                              o.w[3] * (double)slot(m[3], k)) *
-                                             bv;
-                        }
-                    }
+                    for (int k = 0; k < 4; ++k)
+                        accum_ring_pos<QU, CS>(a, P[t], o.ring[k], o.pos[k], o.w[k] * bv, acc[t]);
                 } else {
                     const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-                    int64_t bp;
-                    double bzc, bphic;
-                    nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, bp, bzc, bphic);
-                    const float4 m = load_px<CS>(a.map, bp, a.npix);
-                    if constexpr (QU) {
-                        const double sth_n = sqrt(fmax(0.0, 1.0 - bzc * bzc));
-                        double c2d, s2d;
-                        spin2(bzc, sth_n, bphic, q.z_pts, q.sth_pts, q.phi_pts, c2d, s2d);
-                        const double qv = (double)m.x, uv = (double)m.y;
-                        acc[t][0] += (qv * c2d + uv * s2d) * bv;
-                        acc[t][1] += (-qv * s2d + uv * c2d) * bv;
-#pragma unroll
-                        for (int k = 2; k < CS; ++k)
-                            if (k < nslots) acc[t][k] += (double)slot(m, k) * bv;
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < CS; ++k)
-                            if (k < nslots) acc[t][k] += (double)slot(m, k) * bv;
-                    }
+                    int ring, pos;
+                    nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, ring, pos);
+                    accum_ring_pos<QU, CS>(a, P[t], ring, pos, bv, acc[t]);
                 }
             }
         }
-
 #pragma unroll
         for (int t = 0; t < T; ++t) {
             const int64_t b = b0 + t;
 #pragma unroll
             for (int k = 0; k < kMaxSlots; ++k) {
-                if (k < nslots && k < CS) {
+                if (k < a.nslots && k < CS) {
                     const double sum = wave_sum(acc[t][k]);
                     if (lane == 0 && b < a.B) a.tod[(int64_t)a.row[k] * a.B + b] += sum;
                 }
@@ -337,7 +365,412 @@ __global__ __launch_bounds__(256) void k_tod(const TodArgs a) {
     }
 }
 
-// Per-(b, s) dumps of the index path for bit-exact comparisons.
+// ================================================ out-of-line rare paths
+
+// Four accumulator slots returned by value (no stack array).
+struct Acc4 {
+    double v[kMaxSlots];
+};
+
+// Direct contribution of one map pixel with weight w (spin-2 on slots 0/1
+// when QU).  Kept out of line: the footprint kernel only takes it for the
+// rare neighbour that falls outside its LDS grid.
+template <bool QU, int CS>
+__device__ __forceinline__ Acc4 direct_one(const float* __restrict__ map, int64_t npix, int nslots,
+                                        int64_t p, double z_n, double sth_n, double phi_n,
+                                        double w, double z_pts, double sth_pts,
+                                        double phi_pts) {
+    Acc4 r = {{0.0, 0.0, 0.0, 0.0}};
+    const float4 m = load_px<CS>(map, p, npix);
+    if constexpr (QU) {
+        double c2d, s2d;
+        spin2(z_n, sth_n, phi_n, z_pts, sth_pts, phi_pts, c2d, s2d);
+        const double qv = (double)m.x, uv = (double)m.y;
+        r.v[0] = w * fma(qv, c2d, uv * s2d);
+        r.v[1] = w * fma(uv, c2d, -(qv * s2d));
+#pragma unroll
+        for (int k = 2; k < CS; ++k)
+            if (k < nslots) r.v[k] = w * (double)slot(m, k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < CS; ++k)
+            if (k < nslots) r.v[k] = w * (double)slot(m, k);
+    }
+    return r;
+}
+
+// ========================================================= footprint kernel
+
+constexpr int kFpThreads = 512;
+constexpr int kFpWaves = kFpThreads / 64;
+
+// One footprint ring in LDS.
+struct alignas(16) FpRing {
+    double z, sth, theta, phi0, dphi, inv_dphi, inv_dth;
+    int64_t first;
+    int n, i0, width, off;
+};
+
+struct FpConf {
+    int cap;        // weight-grid entries
+    int max_rings;  // FpRing entries
+};
+
+// Local index of (ring entry e, slot pos in [0, n]) in the weight grid, or -1.
+__device__ __forceinline__ int fp_local(const FpRing& e, int pos) {
+    int d = pos - e.i0;
+    if (d < 0) d += e.n;
+    if (d >= e.n) d -= e.n;
+    return d < e.width ? e.off + d : -1;
+}
+
+// Bilinear neighbours through the global ring table (pole caps, rings
+// outside the layout): LDS cell when laid out, direct contribution otherwise.
+template <bool QU, int CS>
+__device__ __forceinline__ Acc4 fp_slow_bilinear(const TodArgs* __restrict__ ap, double z,
+                                              double phi_w, double bv, const FpRing* rl,
+                                              int ir_lo, int nr, double* W, double z_pts,
+                                              double sth_pts, double phi_pts) {
+    const TodArgs& a = *ap;
+    Nb o;
+    ring_interp(a.nside, a.rings, z, phi_w, o);
+    Acc4 r = {{0.0, 0.0, 0.0, 0.0}};
+    for (int k = 0; k < 4; ++k) {
+        const int lr = o.ring[k] - ir_lo;
+        const int li = (lr >= 0 && lr < nr) ? fp_local(rl[lr], o.pos[k]) : -1;
+        if (li >= 0) {
+            atomicAdd(&W[li], o.w[k] * bv);
+        } else {
+            const RingEntry& e = a.rings[o.ring[k]];
+            const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + o.pos[k], e.z,
+                                              e.sth, e.phi0 + (double)o.pos[k] * e.dphi,
+                                              o.w[k] * bv, z_pts, sth_pts, phi_pts);
+            for (int j = 0; j < kMaxSlots; ++j) r.v[j] += d.v[j];
+        }
+    }
+    return r;
+}
+
+template <int MODE, bool QU, int CS, int MINW>
+__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, const FpConf fc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    FpRing* rings = reinterpret_cast<FpRing*>(smem);
+    double* W = reinterpret_cast<double*>(smem + (size_t)fc.max_rings * sizeof(FpRing));
+    double* red = W + fc.cap;                                           // kFpWaves*kMaxSlots
+    int* ishared = reinterpret_cast<int*>(red + kFpWaves * kMaxSlots);  // 4 ints
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nside = a.nside;
+
+    // XCD-aware sample order: workgroups are dispatched round-robin over the
+    // 8 XCDs, so block bid runs on XCD bid % 8.  Each XCD gets a contiguous
+    // eighth of the samples, so concurrently processed samples on one XCD
+    // have overlapping footprints and share that XCD's L2.
+    const int G = gridDim.x;
+    const int xcd = blockIdx.x & 7;
+    const int per = G >> 3;
+    const int kx = blockIdx.x >> 3;
+    const int64_t chunk = (a.B + 7) / 8;
+    const int64_t bbeg = (int64_t)xcd * chunk;
+    const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
+    if (bbeg + kx >= bend) return;  // uniform over the block
+
+    // ---- beam radius around the beam centre c = R1^-1(boresight) of the
+    // block's first sample.  For a beam prepared around its centre every
+    // sample's rotation maps c onto its boresight; any c yields a correct
+    // footprint disc around R_b c (rotations keep angles), a good c a small one.
+    double cx, cy, cz, cosr;
+    {
+        Sample q;
+        load_sample(a, bbeg + kx, q);
+        const double px = q.bx, py = q.by, pz = q.bz;
+        const double dkv = q.kx * px + q.ky * py + q.kz * pz;
+        cx = px * q.ca - (q.ky * pz - q.kz * py) * q.sa + q.kx * dkv * q.oma;
+        cy = py * q.ca - (q.kz * px - q.kx * pz) * q.sa + q.ky * dkv * q.oma;
+        cz = pz * q.ca - (q.kx * py - q.ky * px) * q.sa + q.kz * dkv * q.oma;
+        const double nn = fma(cx, cx, fma(cy, cy, cz * cz));
+        const double nrm = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
+        cx *= nrm;
+        cy *= nrm;
+        cz = nn > 0.0 ? cz * nrm : 1.0;
+        double mn = 2.0;
+        for (int64_t s = tid; s < a.S; s += kFpThreads) {
+            const double d = (double)a.vec_orig[3 * s] * cx + (double)a.vec_orig[3 * s + 1] * cy +
+                             (double)a.vec_orig[3 * s + 2] * cz;
+            mn = fmin(mn, d);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) mn = fmin(mn, __shfl_xor(mn, off, 64));
+        if (lane == 0) red[wave] = mn;
+        __syncthreads();
+        mn = red[0];
+        for (int w = 1; w < kFpWaves; ++w) mn = fmin(mn, red[w]);
+        __syncthreads();
+        cosr = fmax(-1.0, fmin(1.0, mn));
+    }
+    // Disc radius for the ring range and the per-ring longitude half-width:
+    // beam radius + 2 pixel sizes (bilinear neighbours lie within ~1.5 of the
+    // query) + a rounding allowance.
+    const double r_beam = acos(cosr);
+    const double margin = 2.0 / (double)nside + 1e-6;
+    const double R = fmin(kPi, r_beam + margin);
+    const double cosR = cos(R);
+
+    for (int64_t b = bbeg + kx; b < bend; b += per) {
+        Sample q;
+        load_sample(a, b, q);
+        make_uniform(q);
+        double fx, fy, fz;  // footprint centre = this sample's image of c
+        rotate(q, cx, cy, cz, fx, fy, fz);
+        const double fzc = fmax(-1.0, fmin(1.0, fz));
+        const double fth = acos(fzc);
+        const double fst = sqrt(fmax(0.0, (1.0 - fzc) * (1.0 + fzc)));
+        double fph = atan2(fy, fx);
+        if (fph < 0.0) fph += kTwoPi;
+
+        const double th_lo = fth - R, th_hi = fth + R;
+        int ir_lo = th_lo <= 0.0 ? 1 : ring_above(nside, cos(th_lo));
+        int ir_hi = th_hi >= kPi ? 4 * nside - 1 : ring_above(nside, cos(th_hi)) + 1;
+        ir_lo = ir_lo < 1 ? 1 : ir_lo;
+        ir_hi = ir_hi > 4 * nside - 1 ? 4 * nside - 1 : ir_hi;
+        int nr = ir_hi - ir_lo + 1;
+        if (nr > fc.max_rings || nr < 0) nr = 0;  // no layout: everything goes direct
+
+        // ---- per-ring slot windows (wave 0) and their prefix offsets
+        if (wave == 0) {
+            int carry = 0, laid = 0;
+            for (int r0 = 0; r0 < nr; r0 += 64) {
+                const int r = r0 + lane;
+                int width = 0, i0 = 0;
+                FpRing e;
+                if (r < nr) {
+                    const RingEntry& g = a.rings[ir_lo + r];
+                    e.z = g.z;
+                    e.sth = g.sth;
+                    e.theta = g.theta;
+                    e.phi0 = g.phi0;
+                    e.dphi = g.dphi;
+                    e.inv_dphi = g.inv_dphi;
+                    e.inv_dth = g.inv_dth;
+                    e.first = g.first;
+                    e.n = g.n;
+                    const double den = fst * g.sth;
+                    const double cd = den > 1e-12 ? (cosR - fzc * g.z) / den : -2.0;
+                    if (cd <= -1.0) {
+                        width = g.n;
+                    } else if (cd < 1.0) {
+                        const double dp = acos(cd);
+                        const double wf = 2.0 * dp / g.dphi + 6.0;
+                        if (wf >= (double)g.n) {
+                            width = g.n;
+                        } else {
+                            width = (int)wf;
+                            int f = (int)floor((fph - dp - g.phi0) / g.dphi) - 2;
+                            f %= g.n;
+                            i0 = f < 0 ? f + g.n : f;
+                        }
+                    }
+                }
+                int incl = width;  // inclusive scan of widths across the wave
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const int t = __shfl_up(incl, d, 64);
+                    if (lane >= d) incl += t;
+                }
+                const int off = carry + incl - width;
+                const bool fits = off + width <= fc.cap;
+                int end = (r < nr && fits) ? off + width : 0;
+                if (r < nr) {
+                    e.i0 = i0;
+                    e.width = fits ? width : 0;  // beyond capacity: accumulated directly
+                    e.off = fits ? off : 0;
+                    rings[r] = e;
+                }
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
+                laid = max(laid, end);
+                carry += __shfl(incl, 63, 64);
+            }
+            if (lane == 0) ishared[0] = laid;  // grid cells [0, laid) belong to rings
+        }
+        __syncthreads();
+        const int total = ishared[0];
+        for (int j = tid; j < total; j += kFpThreads) W[j] = 0.0;
+        __syncthreads();
+
+        // ---- phase 1: beam sweep, scatter weights into the LDS grid.
+        // A lean loop handles the common case (both rings laid out, all four
+        // neighbours inside the grid, no transcendental beyond atan2/sqrt);
+        // any beam pixel that needs more is flagged in a per-thread bit mask
+        // and redone by the general code in a second loop, so the hot loop's
+        // register allocation carries none of the rare paths.
+        double accd[kMaxSlots] = {0.0, 0.0, 0.0, 0.0};  // outside-grid sums
+        for (int64_t s0 = 0; s0 < a.S; s0 += (int64_t)kFpThreads * 64) {
+            uint64_t slow = 0;
+            float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
+            int64_t s = s0 + tid;
+            if (s < a.S) {
+                nv0 = a.vec_orig[3 * s];
+                nv1 = a.vec_orig[3 * s + 1];
+                nv2 = a.vec_orig[3 * s + 2];
+                nbv = a.beam_vals[s];
+            }
+            for (int it = 0; it < 64 && s < a.S; ++it, s += kFpThreads) {
+                const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
+                const double bv = (double)nbv;
+                const int64_t sn = s + kFpThreads;
+                if (it + 1 < 64 && sn < a.S) {  // fetch the next beam pixel early
+                    nv0 = a.vec_orig[3 * sn];
+                    nv1 = a.vec_orig[3 * sn + 1];
+                    nv2 = a.vec_orig[3 * sn + 2];
+                    nbv = a.beam_vals[sn];
+                }
+                double vx, vy, vz;
+                rotate(q, v0, v1, v2, vx, vy, vz);
+                const double phi_w = wrap_phi(vy, vx);
+                bool ok = false;
+                if constexpr (MODE == 0) {
+                    const double z = fmax(-1.0, fmin(1.0, vz));
+                    const int ira = ring_above(nside, z);
+                    const int la = ira - ir_lo;
+                    if (ira >= 1 && ira + 1 <= 4 * nside - 1 && la >= 0 && la + 1 < nr) {
+                        const FpRing& A = rings[la];
+                        const FpRing& Bn = rings[la + 1];
+                        const double sq = sqrt(fmax(0.0, (1.0 - z) * (1.0 + z)));
+                        const double x = fma(sq, A.z, -(z * A.sth));
+                        const double xa = phi_w - A.phi0, xb = phi_w - Bn.phi0;
+                        if (fabs(x) < 1e-2 && fabs(xa) >= 0x1p-960 && fabs(xb) >= 0x1p-960) {
+                            const double x2 = x * x;
+                            double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
+                            pp = fma(x2, pp, 3.0 / 40.0);
+                            pp = fma(x2, pp, 1.0 / 6.0);
+                            const double w_below = fma(x * x2, pp, x) * A.inv_dth;
+                            const double w_above = 1.0 - w_below;
+                            double fa, fb;
+                            const int ia = ring_slot(phi_w, A.phi0, A.dphi, A.inv_dphi, A.n, fa);
+                            const int ib = ring_slot(phi_w, Bn.phi0, Bn.dphi, Bn.inv_dphi, Bn.n, fb);
+                            const int l0 = fp_local(A, ia), l1 = fp_local(A, next_slot(ia, A.n));
+                            const int l2 = fp_local(Bn, ib), l3 = fp_local(Bn, next_slot(ib, Bn.n));
+                            if ((l0 | l1 | l2 | l3) >= 0) {
+                                const double wa = w_above * bv, wb = w_below * bv;
+                                atomicAdd(&W[l0], wa * (1.0 - fa));
+                                atomicAdd(&W[l1], wa * fa);
+                                atomicAdd(&W[l2], wb * (1.0 - fb));
+                                atomicAdd(&W[l3], wb * fb);
+                                ok = true;
+                            }
+                        }
+                    }
+                } else {
+                    const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
+                    int ring, pos;
+                    nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
+                    const int lr = ring - ir_lo;
+                    const int li = (lr >= 0 && lr < nr) ? fp_local(rings[lr], pos) : -1;
+                    if (li >= 0) {
+                        atomicAdd(&W[li], bv);
+                        ok = true;
+                    }
+                }
+                if (!ok) slow |= 1ull << it;
+            }
+            // general code for the flagged beam pixels
+            while (slow) {
+                const int it = __builtin_ctzll(slow);
+                slow &= slow - 1;
+                const int64_t ss = s0 + (int64_t)it * kFpThreads + tid;
+                const double v0 = (double)a.vec_orig[3 * ss];
+                const double v1 = (double)a.vec_orig[3 * ss + 1];
+                const double v2 = (double)a.vec_orig[3 * ss + 2];
+                const double bv = (double)a.beam_vals[ss];
+                double vx, vy, vz;
+                rotate(q, v0, v1, v2, vx, vy, vz);
+                const double phi_w = wrap_phi(vy, vx);
+                if constexpr (MODE == 0) {
+                    const Acc4 d = fp_slow_bilinear<QU, CS>(&a, fmax(-1.0, fmin(1.0, vz)), phi_w, bv,
+                                                            rings, ir_lo, nr, W, q.z_pts,
+                                                            q.sth_pts, q.phi_pts);
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
+                } else {
+                    const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
+                    int ring, pos;
+                    nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
+                    const RingEntry& e = a.rings[ring];
+                    const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + pos, e.z,
+                                                      e.sth, e.phi0 + (double)pos * e.dphi, bv,
+                                                      q.z_pts, q.sth_pts, q.phi_pts);
+#pragma unroll
+                    for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- phase 2: footprint sweep, spin-2 + one gather per map pixel
+        double acc[kMaxSlots] = {accd[0], accd[1], accd[2], accd[3]};
+        int r = 0;
+        for (int j = tid; j < total; j += kFpThreads) {
+            while (r + 1 < nr && j >= rings[r].off + rings[r].width) ++r;
+            const double wsum = W[j];
+            if (wsum == 0.0) continue;
+            const FpRing& e = rings[r];
+            int pos = e.i0 + (j - e.off);
+            if (pos >= e.n) pos -= e.n;
+            const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
+            if constexpr (QU) {
+                const double phi_n = e.phi0 + (double)pos * e.dphi;
+                double c2d, s2d;
+                if (q.phi_pts == phi_n && q.sth_pts == e.sth && q.z_pts == e.z) {
+                    c2d = 1.0;
+                    s2d = 0.0;
+                } else {
+                    double h = (q.phi_pts - phi_n) * 0.5;
+                    if (h > 0.5 * kPi) h -= kPi;
+                    if (h < -0.5 * kPi) h += kPi;
+                    double sh, ch;
+                    sincos_half(h, sh, ch);
+                    spin2_from_half(spin2_ring(e.z, e.sth, q.z_pts, q.sth_pts), sh, ch, c2d, s2d);
+                }
+                const double qv = (double)m.x, uv = (double)m.y;
+                acc[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc[0]);
+                acc[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc[1]);
+#pragma unroll
+                for (int k = 2; k < CS; ++k)
+                    if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < CS; ++k)
+                    if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
+            }
+        }
+
+        // ---- block reduction and write-out
+#pragma unroll
+        for (int k = 0; k < kMaxSlots; ++k) {
+            if (k < CS) {
+                const double v = wave_sum(acc[k]);
+                if (lane == 0) red[wave * kMaxSlots + k] = v;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            // one thread, slots in order: two slots may share a row (c_q == c_u)
+            for (int k = 0; k < kMaxSlots && k < CS; ++k) {
+                if (k >= a.nslots) break;
+                double sum = 0.0;
+                for (int w = 0; w < kFpWaves; ++w) sum += red[w * kMaxSlots + k];
+                a.tod[(int64_t)a.row[k] * a.B + b] += sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ================================================== index-path dump kernels
+
 __global__ __launch_bounds__(256) void k_bilinear_indices(const TodArgs a, int64_t b0,
                                                            int64_t nb, int64_t* pix,
                                                            double* wgt) {
@@ -347,17 +780,14 @@ __global__ __launch_bounds__(256) void k_bilinear_indices(const TodArgs a, int64
     Sample q;
     load_sample(a, b, q);
     double vx, vy, vz;
-    rodrigues((double)a.vec_orig[3 * s], (double)a.vec_orig[3 * s + 1],
-              (double)a.vec_orig[3 * s + 2], q.kx, q.ky, q.kz, q.ca, q.sa, q.bx, q.by, q.bz,
-              q.cp, q.sp, vx, vy, vz);
-    double phi_w = atan2(vy, vx);
-    if (phi_w < 0.0) phi_w += kTwoPi;
-    const double z = fmax(-1.0, fmin(1.0, vz));
-    Bilin o;
-    ring_interp(a.nside, a.rings, z, phi_w, o);
+    rotate(q, (double)a.vec_orig[3 * s], (double)a.vec_orig[3 * s + 1],
+           (double)a.vec_orig[3 * s + 2], vx, vy, vz);
+    const double phi_w = wrap_phi(vy, vx);
+    Nb o;
+    ring_interp(a.nside, a.rings, fmax(-1.0, fmin(1.0, vz)), phi_w, o);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        pix[4 * i + k] = o.p[k];
+        pix[4 * i + k] = a.rings[o.ring[k]].first + o.pos[k];
         wgt[4 * i + k] = o.w[k];
     }
 }
@@ -370,16 +800,13 @@ __global__ __launch_bounds__(256) void k_nearest_indices(const TodArgs a, int64_
     Sample q;
     load_sample(a, b, q);
     double vx, vy, vz;
-    rodrigues((double)a.vec_orig[3 * s], (double)a.vec_orig[3 * s + 1],
-              (double)a.vec_orig[3 * s + 2], q.kx, q.ky, q.kz, q.ca, q.sa, q.bx, q.by, q.bz,
-              q.cp, q.sp, vx, vy, vz);
-    double phi_w = atan2(vy, vx);
-    if (phi_w < 0.0) phi_w += kTwoPi;
+    rotate(q, (double)a.vec_orig[3 * s], (double)a.vec_orig[3 * s + 1],
+           (double)a.vec_orig[3 * s + 2], vx, vy, vz);
+    const double phi_w = wrap_phi(vy, vx);
     const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-    int64_t bp;
-    double bzc, bphic;
-    nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, bp, bzc, bphic);
-    pix[i] = bp;
+    int ring, pos;
+    nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, ring, pos);
+    pix[i] = a.rings[ring].first + pos;
 }
 
 // Field-major (C, npix) -> pixel-major (npix, CS) with slot k taken from
@@ -393,6 +820,8 @@ __global__ __launch_bounds__(256) void k_pack_map(const float* __restrict__ src,
     for (int k = 0; k < CS; ++k)
         dst[CS * i + k] = perm[k] >= 0 ? src[(int64_t)perm[k] * npix + i] : 0.f;
 }
+
+// ============================================================ host helpers
 
 int slots_stride(int nslots) { return nslots <= 1 ? 1 : (nslots == 2 ? 2 : 4); }
 
@@ -435,31 +864,75 @@ int slot_layout(int C, int c_q, int c_u, bool* qu, int perm[kMaxSlots], int row[
     return TODG_OK;
 }
 
+// Footprint-kernel LDS configuration: rings and weight-grid entries
+// (~53 KB per workgroup: two workgroups of 8 waves per CU).
+constexpr int kFpMaxRings = 192;
+constexpr int kFpCap = 4608;
+
+size_t fp_smem_bytes(const FpConf& fc) {
+    return (size_t)fc.max_rings * sizeof(FpRing) + (size_t)fc.cap * sizeof(double) +
+           (size_t)kFpWaves * kMaxSlots * sizeof(double) + 16;
+}
+
 template <int MODE, bool QU, int CS>
-hipError_t launch_cs(const TodArgs& a, int T, int blocks, hipStream_t st) {
+hipError_t launch_direct_cs(const TodArgs& a, int T, hipStream_t st) {
+    const int64_t ntiles = (a.B + T - 1) / T;
+    const int64_t waves_cap = (int64_t)cu_count() * 32;
+    const int64_t waves = ntiles < waves_cap ? ntiles : waves_cap;
+    const int blocks = (int)((waves + 3) / 4);
     switch (T) {
-        case 1: hipLaunchKernelGGL((k_tod<MODE, QU, CS, 1>), dim3(blocks), dim3(256), 0, st, a); break;
-        case 2: hipLaunchKernelGGL((k_tod<MODE, QU, CS, 2>), dim3(blocks), dim3(256), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_tod<MODE, QU, CS, 4>), dim3(blocks), dim3(256), 0, st, a); break;
+        case 1: hipLaunchKernelGGL((k_tod_direct<MODE, QU, CS, 1>), dim3(blocks), dim3(256), 0, st, a); break;
+        case 2: hipLaunchKernelGGL((k_tod_direct<MODE, QU, CS, 2>), dim3(blocks), dim3(256), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_tod_direct<MODE, QU, CS, 4>), dim3(blocks), dim3(256), 0, st, a); break;
     }
     return hipGetLastError();
 }
 
+// MINW = waves per SIMD the kernel is compiled for: 2 (256 VGPRs, no spills,
+// one workgroup per CU) or 4 (128 VGPRs, two workgroups per CU).
+template <int MODE, bool QU, int CS, int MINW>
+hipError_t launch_fp_cs(const TodArgs& a, hipStream_t st) {
+    const FpConf fc{kFpCap, kFpMaxRings};
+    const size_t smem = fp_smem_bytes(fc);
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)k_tod_fp<MODE, QU, CS, MINW>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    });
+    // Workgroups resident per CU (LDS allows two), a multiple of 8 for the
+    // XCD mapping, no more than the samples need.
+    int64_t blocks = (int64_t)cu_count() * (MINW >= 4 ? 2 : 1) * 2;
+    const int64_t need = ((a.B + 7) / 8) * 8;
+    if (blocks > need) blocks = need;
+    blocks = ((blocks + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_tod_fp<MODE, QU, CS, MINW>), dim3((unsigned)blocks), dim3(kFpThreads),
+                       smem, st, a, fc);
+    return hipGetLastError();
+}
+
+template <int MODE, bool QU, int CS>
+hipError_t launch_cs(const TodArgs& a, int variant, hipStream_t st) {
+    if (variant == 1 || variant == 2 || variant == 4)
+        return launch_direct_cs<MODE, QU, CS>(a, variant, st);
+    if (variant == 8) return launch_fp_cs<MODE, QU, CS, 4>(a, st);
+    return launch_fp_cs<MODE, QU, CS, 2>(a, st);
+}
+
 template <int MODE>
-hipError_t launch_mode(const TodArgs& a, bool qu, int CS, int T, int blocks, hipStream_t st) {
+hipError_t launch_mode(const TodArgs& a, bool qu, int CS, int variant, hipStream_t st) {
     if (qu) {
-        if (CS == 2) return launch_cs<MODE, true, 2>(a, T, blocks, st);
-        return launch_cs<MODE, true, 4>(a, T, blocks, st);
+        if (CS == 2) return launch_cs<MODE, true, 2>(a, variant, st);
+        return launch_cs<MODE, true, 4>(a, variant, st);
     }
-    if (CS == 1) return launch_cs<MODE, false, 1>(a, T, blocks, st);
-    if (CS == 2) return launch_cs<MODE, false, 2>(a, T, blocks, st);
-    return launch_cs<MODE, false, 4>(a, T, blocks, st);
+    if (CS == 1) return launch_cs<MODE, false, 1>(a, variant, st);
+    if (CS == 2) return launch_cs<MODE, false, 2>(a, variant, st);
+    return launch_cs<MODE, false, 4>(a, variant, st);
 }
 
 int fill_args(const float* vec_orig, const float* axes, const float* cos_a, const float* sin_a,
               const float* ax_pts, const float* cos_p, const float* sin_p, int64_t nside,
               const float* map_packed, int C, const float* beam_vals, int64_t B, int64_t S,
-              double* tod, int c_q, int c_u, TodArgs* a, bool* qu, int* CS) {
+              double* tod, int c_q, int c_u, bool need_map, TodArgs* a, bool* qu, int* CS) {
     if (!is_pow2(nside) || nside > 8192)
         return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
     if (B < 0 || S < 0) return fail(TODG_ERR_ARG, "B and S must be non-negative");
@@ -469,7 +942,7 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
     if (rc) return rc;
     if (B > 0 && S > 0 &&
         (!vec_orig || !axes || !cos_a || !sin_a || !ax_pts || !cos_p || !sin_p ||
-         !map_packed || !beam_vals || !tod))
+         (need_map && (!map_packed || !beam_vals || !tod))))
         return fail(TODG_ERR_ARG, "null pointer argument");
     const RingEntry* rings = nullptr;
     rc = get_ring_table(nside, &rings);
@@ -497,22 +970,17 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
 int run_tod(int mode, const float* vec_orig, const float* axes, const float* cos_a,
             const float* sin_a, const float* ax_pts, const float* cos_p, const float* sin_p,
             int64_t nside, const float* map_packed, int C, const float* beam_vals, int64_t B,
-            int64_t S, double* tod, int c_q, int c_u, int tile, void* stream) {
+            int64_t S, double* tod, int c_q, int c_u, int variant, void* stream) {
     TodArgs a;
     bool qu;
     int CS;
     int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
-                       beam_vals, B, S, tod, c_q, c_u, &a, &qu, &CS);
+                       beam_vals, B, S, tod, c_q, c_u, true, &a, &qu, &CS);
     if (rc) return rc;
     if (B == 0 || S == 0) return TODG_OK;
-    const int T = tile == 1 || tile == 2 || tile == 4 ? tile : 2;
-    const int64_t ntiles = (B + T - 1) / T;
-    const int64_t waves_cap = (int64_t)cu_count() * 32;  // 8 waves/SIMD-ish residency
-    int64_t waves = ntiles < waves_cap ? ntiles : waves_cap;
-    const int blocks = (int)((waves + 3) / 4);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipError_t e = mode == 0 ? launch_mode<0>(a, qu, CS, T, blocks, st)
-                             : launch_mode<1>(a, qu, CS, T, blocks, st);
+    hipError_t e = mode == 0 ? launch_mode<0>(a, qu, CS, variant, st)
+                             : launch_mode<1>(a, qu, CS, variant, st);
     if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TODG_OK;
 }
@@ -565,18 +1033,18 @@ int todg_gather_accum_fused(const float* vec_orig, const float* axes, const floa
                             const float* sin_a, const float* ax_pts, const float* cos_p,
                             const float* sin_p, int64_t nside, const float* map_packed, int C,
                             const float* beam_vals, int64_t B, int64_t S, double* tod, int c_q,
-                            int c_u, int tile, void* stream) {
+                            int c_u, int variant, void* stream) {
     return run_tod(0, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
-                   beam_vals, B, S, tod, c_q, c_u, tile, stream);
+                   beam_vals, B, S, tod, c_q, c_u, variant, stream);
 }
 
 int todg_gather_accum_nearest(const float* vec_orig, const float* axes, const float* cos_a,
                               const float* sin_a, const float* ax_pts, const float* cos_p,
                               const float* sin_p, int64_t nside, const float* map_packed, int C,
                               const float* beam_vals, int64_t B, int64_t S, double* tod,
-                              int c_q, int c_u, int tile, void* stream) {
+                              int c_q, int c_u, int variant, void* stream) {
     return run_tod(1, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
-                   beam_vals, B, S, tod, c_q, c_u, tile, stream);
+                   beam_vals, B, S, tod, c_q, c_u, variant, stream);
 }
 
 int todg_bilinear_indices(const float* vec_orig, const float* axes, const float* cos_a,
@@ -587,9 +1055,8 @@ int todg_bilinear_indices(const float* vec_orig, const float* axes, const float*
     TodArgs a;
     bool qu;
     int CS;
-    static const float dummy_map[4] = {0, 0, 0, 0};
-    int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, dummy_map, 1,
-                       vec_orig, B, S, reinterpret_cast<double*>(pix), -1, -1, &a, &qu, &CS);
+    int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, nullptr, 1,
+                       nullptr, B, S, nullptr, -1, -1, false, &a, &qu, &CS);
     if (rc) return rc;
     const int64_t n = nb * S;
     if (n == 0) return TODG_OK;
@@ -608,9 +1075,8 @@ int todg_nearest_indices(const float* vec_orig, const float* axes, const float* 
     TodArgs a;
     bool qu;
     int CS;
-    static const float dummy_map[4] = {0, 0, 0, 0};
-    int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, dummy_map, 1,
-                       vec_orig, B, S, reinterpret_cast<double*>(pix), -1, -1, &a, &qu, &CS);
+    int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, nullptr, 1,
+                       nullptr, B, S, nullptr, -1, -1, false, &a, &qu, &CS);
     if (rc) return rc;
     const int64_t n = nb * S;
     if (n == 0) return TODG_OK;

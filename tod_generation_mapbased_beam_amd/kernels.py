@@ -16,11 +16,13 @@ import torch
 from . import _lib
 from .device import device_map_for, stream_handle, to_dev
 
-DEFAULT_TILE = 2
+# variant 0: footprint kernel (per-sample LDS weight grid, default);
+# 1, 2, 4: direct kernel with that many samples per wavefront.
+DEFAULT_VARIANT = 0
 
 
 def _run(entry, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_stacked,
-         beam_vals, B, S, tod, c_q, c_u, tile):
+         beam_vals, B, S, tod, c_q, c_u, variant):
     B = int(B)
     S = int(S)
     c_q = int(c_q)
@@ -49,7 +51,7 @@ def _run(entry, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_st
         entry(
             _lib.ptr(v), _lib.ptr(ax), _lib.ptr(ca), _lib.ptr(sa), _lib.ptr(bp), _lib.ptr(cp),
             _lib.ptr(sp), int(nside), _lib.ptr(dmap.packed), dmap.C, _lib.ptr(bv), B, S,
-            _lib.ptr(t), c_q, c_u, int(tile), stream_handle(),
+            _lib.ptr(t), c_q, c_u, int(variant), stream_handle(),
         )
     )
     if host_tod is not None:
@@ -58,18 +60,18 @@ def _run(entry, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_st
 
 
 def _gather_accum_fused_jit(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside,
-                            mp_stacked, beam_vals, B, S, tod, c_q=-1, c_u=-1, tile=DEFAULT_TILE):
+                            mp_stacked, beam_vals, B, S, tod, c_q=-1, c_u=-1, variant=DEFAULT_VARIANT):
     """Bilinear + spin-2 beam TOD on the GPU (tod_bilinear.py:205-477)."""
     return _run(_lib.lib().todg_gather_accum_fused, vec_orig, axes, cos_a, sin_a, ax_pts,
-                cos_p, sin_p, nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, tile)
+                cos_p, sin_p, nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, variant)
 
 
 def _gather_accum_nearest_jit(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside,
                               mp_stacked, beam_vals, B, S, tod, c_q=-1, c_u=-1,
-                              tile=DEFAULT_TILE):
+                              variant=DEFAULT_VARIANT):
     """Nearest-pixel + spin-2 beam TOD on the GPU (tod_nearest.py:21-158)."""
     return _run(_lib.lib().todg_gather_accum_nearest, vec_orig, axes, cos_a, sin_a, ax_pts,
-                cos_p, sin_p, nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, tile)
+                cos_p, sin_p, nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, variant)
 
 
 def bilinear_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, B, S, b0, nb):

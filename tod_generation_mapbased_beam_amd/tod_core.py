@@ -29,7 +29,7 @@ def _qu_slots(comp_indices):
 
 
 def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mode="bilinear",
-                   tile=kernels.DEFAULT_TILE):
+                   variant=kernels.DEFAULT_VARIANT):
     """Beam-weighted sky samples of one beam entry for a batch of pointings.
 
     Args mirror tod_core.py:52-96: ``data`` needs ``vec_orig (S,3)``,
@@ -72,7 +72,7 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
 
     tod_arr = torch.zeros((C, B), dtype=torch.float64, device=device())
     fn = kernels._gather_accum_nearest_jit if use_nearest else kernels._gather_accum_fused_jit
-    fn(vec_orig, *params, nside, mp_stacked, beam_vals, B, S, tod_arr, c_q, c_u, tile=tile)
+    fn(vec_orig, *params, nside, mp_stacked, beam_vals, B, S, tod_arr, c_q, c_u, variant=variant)
     out32 = tod_arr.to(torch.float32)
     if on_device:
         return {comp: out32[i] for i, comp in enumerate(comp_indices)}
