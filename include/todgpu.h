@@ -79,6 +79,33 @@ int todg_gather_accum_nearest(const float *vec_orig, const float *axes, const fl
                               int C, const float *beam_vals, int64_t B, int64_t S,
                               double *tod, int c_q, int c_u, int variant, void *stream);
 
+/* One detector (or one detector's scan chunk) of a batched launch.  All
+ * pointers are device pointers with the layouts above; `first` is the
+ * global index of this item's first sample, i.e. the sum of B over the
+ * items before it (items are ordered by `first`). */
+typedef struct todg_work_item {
+    const float *vec_orig, *beam_vals;
+    const float *axes, *cos_a, *sin_a, *ax_pts, *cos_p, *sin_p;
+    double *tod;
+    int64_t B, S, first;
+} todg_work_item;
+
+/* Many detectors with their own beams (uneven S) and scan chunks (uneven B)
+ * in one persistent launch (the reference calls beam_tod_batch once per
+ * beam entry per batch: sample_based_tod_generation_gridint.py:284-296).
+ * mode 0 = bilinear, 1 = nearest; items: device array of n_items; the map,
+ * C and c_q/c_u are shared.  chunk = samples per work-queue claim (1..65536);
+ * at most 2^33 samples per launch.
+ * queue: device scratch of >= 8 unsigned ints (zeroed by the call itself,
+ * so a captured graph replays correctly; concurrent calls need separate
+ * queues).  item_status: optional device int[n_items], bit 0 set for an
+ * item with a non-finite pointing operand (that sample is skipped).
+ * variant: 0 or 8 as for todg_gather_accum_fused. */
+int todg_gather_accum_batch(int mode, const todg_work_item *items, int n_items,
+                            int64_t total_samples, int64_t nside, const float *map_packed,
+                            int C, int c_q, int c_u, int chunk, unsigned int *queue,
+                            int *item_status, int variant, void *stream);
+
 /* Index-path dumps for samples [b0, b0+nb): the four bilinear neighbours and
  * weights of every (b, s) as evaluated inside todg_gather_accum_fused
  * (pix/wgt: (nb, S, 4) int64/float64), i.e. _rodrigues_apply_one_jit +

@@ -163,3 +163,16 @@ def test_synthesised_map_spectrum():
     f = inputs.synthesize_map(nside, [a_full], lmax)[0]
     power = np.sum(np.abs(a[:, 0]) ** 2) + 2 * np.sum(np.abs(a[:, 1:]) ** 2)
     assert abs(np.sum(f * f) * 4 * math.pi / npix / power - 1) < 0.01
+
+
+def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
+    from tod_generation_mapbased_beam_amd import autotune
+
+    monkeypatch.setenv(autotune.CACHE_ENV, str(tmp_path / "tune.json"))
+    k = autotune.workload_key("single", 2048, 6364, "bilinear", 3, True)
+    assert "S~8192" in k and "nside=2048" in k
+    assert autotune.lookup(k) is None
+    assert autotune.tuned_variant(2048, 6364, "bilinear", 3, True, default=0) == 0
+    autotune._store(k, {"variant": 8})
+    assert autotune.tuned_variant(2048, 6000, "bilinear", 3, True) == 8  # same bucket
+    assert autotune.tuned_variant(2048, 9000, "bilinear", 3, True) == 0  # other bucket

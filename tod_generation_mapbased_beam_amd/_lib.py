@@ -52,6 +52,9 @@ SIGNATURES = {
         [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i64, _p, _p],
     ),
     "todg_ring_table": (_i32, [_i64, _p, _p, _p, _p, _p, _p]),
+    "todg_gather_accum_batch": (
+        _i32, [_i32, _p, _i32, _i64, _i64, _p, _i32, _i32, _i32, _i32, _p, _p, _i32, _p],
+    ),
 }
 
 

@@ -451,40 +451,42 @@ __device__ __forceinline__ Acc4 fp_slow_bilinear(const TodArgs* __restrict__ ap,
     return r;
 }
 
-template <int MODE, bool QU, int CS, int MINW>
-__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, const FpConf fc) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    FpRing* rings = reinterpret_cast<FpRing*>(smem);
-    double* W = reinterpret_cast<double*>(smem + (size_t)fc.max_rings * sizeof(FpRing));
-    double* red = W + fc.cap;                                           // kFpWaves*kMaxSlots
-    int* ishared = reinterpret_cast<int*>(red + kFpWaves * kMaxSlots);  // 4 ints
+// LDS carve-up of the footprint kernels.
+struct FpLds {
+    FpRing* rings;  // fc.max_rings ring descriptors
+    double* W;      // fc.cap weight-grid cells
+    double* red;    // kFpWaves * kMaxSlots reduction slots
+    int* ishared;   // 4 ints
+};
 
+__device__ __forceinline__ FpLds fp_lds(unsigned char* smem, const FpConf& fc) {
+    FpLds L;
+    L.rings = reinterpret_cast<FpRing*>(smem);
+    L.W = reinterpret_cast<double*>(smem + (size_t)fc.max_rings * sizeof(FpRing));
+    L.red = L.W + fc.cap;
+    L.ishared = reinterpret_cast<int*>(L.red + kFpWaves * kMaxSlots);
+    return L;
+}
+
+// Beam centre c (unit vector in the beam frame) and the footprint disc
+// radius R around its image R_b c.
+struct BeamFrame {
+    double cx, cy, cz, R, cosR;
+};
+
+// All threads of the block: the beam centre c = R1^-1(boresight) of sample
+// b0 and the beam radius about it.  For a beam prepared around its centre
+// every sample's rotation maps c onto its boresight; any c yields a correct
+// footprint disc around R_b c (rotations keep angles), a good c a small one.
+__device__ __forceinline__ BeamFrame fp_beam_frame(const TodArgs& a, int64_t b0, const FpLds& L) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    const int nside = a.nside;
-
-    // XCD-aware sample order: workgroups are dispatched round-robin over the
-    // 8 XCDs, so block bid runs on XCD bid % 8.  Each XCD gets a contiguous
-    // eighth of the samples, so concurrently processed samples on one XCD
-    // have overlapping footprints and share that XCD's L2.
-    const int G = gridDim.x;
-    const int xcd = blockIdx.x & 7;
-    const int per = G >> 3;
-    const int kx = blockIdx.x >> 3;
-    const int64_t chunk = (a.B + 7) / 8;
-    const int64_t bbeg = (int64_t)xcd * chunk;
-    const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
-    if (bbeg + kx >= bend) return;  // uniform over the block
-
-    // ---- beam radius around the beam centre c = R1^-1(boresight) of the
-    // block's first sample.  For a beam prepared around its centre every
-    // sample's rotation maps c onto its boresight; any c yields a correct
-    // footprint disc around R_b c (rotations keep angles), a good c a small one.
+    double* red = L.red;
     double cx, cy, cz, cosr;
     {
         Sample q;
-        load_sample(a, bbeg + kx, q);
+        load_sample(a, b0, q);
         const double px = q.bx, py = q.by, pz = q.bz;
         const double dkv = q.kx * px + q.ky * py + q.kz * pz;
         cx = px * q.ca - (q.ky * pz - q.kz * py) * q.sa + q.kx * dkv * q.oma;
@@ -513,260 +515,413 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, co
     // Disc radius for the ring range and the per-ring longitude half-width:
     // beam radius + 2 pixel sizes (bilinear neighbours lie within ~1.5 of the
     // query) + a rounding allowance.
-    const double r_beam = acos(cosr);
-    const double margin = 2.0 / (double)nside + 1e-6;
-    const double R = fmin(kPi, r_beam + margin);
-    const double cosR = cos(R);
+    BeamFrame f;
+    f.cx = cx;
+    f.cy = cy;
+    f.cz = cz;
+    f.R = fmin(kPi, acos(cosr) + 2.0 / (double)a.nside + 1e-6);
+    f.cosR = cos(f.R);
+    return f;
+}
 
-    for (int64_t b = bbeg + kx; b < bend; b += per) {
-        Sample q;
-        load_sample(a, b, q);
-        make_uniform(q);
-        double fx, fy, fz;  // footprint centre = this sample's image of c
-        rotate(q, cx, cy, cz, fx, fy, fz);
-        const double fzc = fmax(-1.0, fmin(1.0, fz));
-        const double fth = acos(fzc);
-        const double fst = sqrt(fmax(0.0, (1.0 - fzc) * (1.0 + fzc)));
-        double fph = atan2(fy, fx);
-        if (fph < 0.0) fph += kTwoPi;
+// All threads of the block: one sample's footprint layout, beam sweep,
+// footprint sweep and tod update.
+template <int MODE, bool QU, int CS>
+__device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const BeamFrame& bf,
+                                          const FpConf& fc, const FpLds& L) {
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int nside = a.nside;
+    FpRing* rings = L.rings;
+    double* W = L.W;
+    double* red = L.red;
+    int* ishared = L.ishared;
+    const double cx = bf.cx, cy = bf.cy, cz = bf.cz, R = bf.R, cosR = bf.cosR;
+    Sample q;
+    load_sample(a, b, q);
+    make_uniform(q);
+    double fx, fy, fz;  // footprint centre = this sample's image of c
+    rotate(q, cx, cy, cz, fx, fy, fz);
+    const double fzc = fmax(-1.0, fmin(1.0, fz));
+    const double fth = acos(fzc);
+    const double fst = sqrt(fmax(0.0, (1.0 - fzc) * (1.0 + fzc)));
+    double fph = atan2(fy, fx);
+    if (fph < 0.0) fph += kTwoPi;
 
-        const double th_lo = fth - R, th_hi = fth + R;
-        int ir_lo = th_lo <= 0.0 ? 1 : ring_above(nside, cos(th_lo));
-        int ir_hi = th_hi >= kPi ? 4 * nside - 1 : ring_above(nside, cos(th_hi)) + 1;
-        ir_lo = ir_lo < 1 ? 1 : ir_lo;
-        ir_hi = ir_hi > 4 * nside - 1 ? 4 * nside - 1 : ir_hi;
-        int nr = ir_hi - ir_lo + 1;
-        if (nr > fc.max_rings || nr < 0) nr = 0;  // no layout: everything goes direct
+    const double th_lo = fth - R, th_hi = fth + R;
+    int ir_lo = th_lo <= 0.0 ? 1 : ring_above(nside, cos(th_lo));
+    int ir_hi = th_hi >= kPi ? 4 * nside - 1 : ring_above(nside, cos(th_hi)) + 1;
+    ir_lo = ir_lo < 1 ? 1 : ir_lo;
+    ir_hi = ir_hi > 4 * nside - 1 ? 4 * nside - 1 : ir_hi;
+    int nr = ir_hi - ir_lo + 1;
+    if (nr > fc.max_rings || nr < 0) nr = 0;  // no layout: everything goes direct
 
-        // ---- per-ring slot windows (wave 0) and their prefix offsets
-        if (wave == 0) {
-            int carry = 0, laid = 0;
-            for (int r0 = 0; r0 < nr; r0 += 64) {
-                const int r = r0 + lane;
-                int width = 0, i0 = 0;
-                FpRing e;
-                if (r < nr) {
-                    const RingEntry& g = a.rings[ir_lo + r];
-                    e.z = g.z;
-                    e.sth = g.sth;
-                    e.theta = g.theta;
-                    e.phi0 = g.phi0;
-                    e.dphi = g.dphi;
-                    e.inv_dphi = g.inv_dphi;
-                    e.inv_dth = g.inv_dth;
-                    e.first = g.first;
-                    e.n = g.n;
-                    const double den = fst * g.sth;
-                    const double cd = den > 1e-12 ? (cosR - fzc * g.z) / den : -2.0;
-                    if (cd <= -1.0) {
+    // ---- per-ring slot windows (wave 0) and their prefix offsets
+    if (wave == 0) {
+        int carry = 0, laid = 0;
+        for (int r0 = 0; r0 < nr; r0 += 64) {
+            const int r = r0 + lane;
+            int width = 0, i0 = 0;
+            FpRing e;
+            if (r < nr) {
+                const RingEntry& g = a.rings[ir_lo + r];
+                e.z = g.z;
+                e.sth = g.sth;
+                e.theta = g.theta;
+                e.phi0 = g.phi0;
+                e.dphi = g.dphi;
+                e.inv_dphi = g.inv_dphi;
+                e.inv_dth = g.inv_dth;
+                e.first = g.first;
+                e.n = g.n;
+                const double den = fst * g.sth;
+                const double cd = den > 1e-12 ? (cosR - fzc * g.z) / den : -2.0;
+                if (cd <= -1.0) {
+                    width = g.n;
+                } else if (cd < 1.0) {
+                    const double dp = acos(cd);
+                    const double wf = 2.0 * dp / g.dphi + 6.0;
+                    if (wf >= (double)g.n) {
                         width = g.n;
-                    } else if (cd < 1.0) {
-                        const double dp = acos(cd);
-                        const double wf = 2.0 * dp / g.dphi + 6.0;
-                        if (wf >= (double)g.n) {
-                            width = g.n;
-                        } else {
-                            width = (int)wf;
-                            int f = (int)floor((fph - dp - g.phi0) / g.dphi) - 2;
-                            f %= g.n;
-                            i0 = f < 0 ? f + g.n : f;
-                        }
+                    } else {
+                        width = (int)wf;
+                        int f = (int)floor((fph - dp - g.phi0) / g.dphi) - 2;
+                        f %= g.n;
+                        i0 = f < 0 ? f + g.n : f;
                     }
                 }
-                int incl = width;  // inclusive scan of widths across the wave
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int t = __shfl_up(incl, d, 64);
-                    if (lane >= d) incl += t;
-                }
-                const int off = carry + incl - width;
-                const bool fits = off + width <= fc.cap;
-                int end = (r < nr && fits) ? off + width : 0;
-                if (r < nr) {
-                    e.i0 = i0;
-                    e.width = fits ? width : 0;  // beyond capacity: accumulated directly
-                    e.off = fits ? off : 0;
-                    rings[r] = e;
-                }
-#pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
-                laid = max(laid, end);
-                carry += __shfl(incl, 63, 64);
             }
-            if (lane == 0) ishared[0] = laid;  // grid cells [0, laid) belong to rings
+            int incl = width;  // inclusive scan of widths across the wave
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int t = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += t;
+            }
+            const int off = carry + incl - width;
+            const bool fits = off + width <= fc.cap;
+            int end = (r < nr && fits) ? off + width : 0;
+            if (r < nr) {
+                e.i0 = i0;
+                e.width = fits ? width : 0;  // beyond capacity: accumulated directly
+                e.off = fits ? off : 0;
+                rings[r] = e;
+            }
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
+            laid = max(laid, end);
+            carry += __shfl(incl, 63, 64);
         }
-        __syncthreads();
-        const int total = ishared[0];
-        for (int j = tid; j < total; j += kFpThreads) W[j] = 0.0;
-        __syncthreads();
-
-        // ---- phase 1: beam sweep, scatter weights into the LDS grid.
-        // A lean loop handles the common case (both rings laid out, all four
-        // neighbours inside the grid, no transcendental beyond atan2/sqrt);
-        // any beam pixel that needs more is flagged in a per-thread bit mask
-        // and redone by the general code in a second loop, so the hot loop's
-        // register allocation carries none of the rare paths.
-        double accd[kMaxSlots] = {0.0, 0.0, 0.0, 0.0};  // outside-grid sums
-        for (int64_t s0 = 0; s0 < a.S; s0 += (int64_t)kFpThreads * 64) {
-            uint64_t slow = 0;
-            float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
-            int64_t s = s0 + tid;
-            if (s < a.S) {
-                nv0 = a.vec_orig[3 * s];
-                nv1 = a.vec_orig[3 * s + 1];
-                nv2 = a.vec_orig[3 * s + 2];
-                nbv = a.beam_vals[s];
-            }
-            for (int it = 0; it < 64 && s < a.S; ++it, s += kFpThreads) {
-                const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
-                const double bv = (double)nbv;
-                const int64_t sn = s + kFpThreads;
-                if (it + 1 < 64 && sn < a.S) {  // fetch the next beam pixel early
-                    nv0 = a.vec_orig[3 * sn];
-                    nv1 = a.vec_orig[3 * sn + 1];
-                    nv2 = a.vec_orig[3 * sn + 2];
-                    nbv = a.beam_vals[sn];
-                }
-                double vx, vy, vz;
-                rotate(q, v0, v1, v2, vx, vy, vz);
-                const double phi_w = wrap_phi(vy, vx);
-                bool ok = false;
-                if constexpr (MODE == 0) {
-                    const double z = fmax(-1.0, fmin(1.0, vz));
-                    const int ira = ring_above(nside, z);
-                    const int la = ira - ir_lo;
-                    if (ira >= 1 && ira + 1 <= 4 * nside - 1 && la >= 0 && la + 1 < nr) {
-                        const FpRing& A = rings[la];
-                        const FpRing& Bn = rings[la + 1];
-                        const double sq = sqrt(fmax(0.0, (1.0 - z) * (1.0 + z)));
-                        const double x = fma(sq, A.z, -(z * A.sth));
-                        const double xa = phi_w - A.phi0, xb = phi_w - Bn.phi0;
-                        if (fabs(x) < 1e-2 && fabs(xa) >= 0x1p-960 && fabs(xb) >= 0x1p-960) {
-                            const double x2 = x * x;
-                            double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
-                            pp = fma(x2, pp, 3.0 / 40.0);
-                            pp = fma(x2, pp, 1.0 / 6.0);
-                            const double w_below = fma(x * x2, pp, x) * A.inv_dth;
-                            const double w_above = 1.0 - w_below;
-                            double fa, fb;
-                            const int ia = ring_slot(phi_w, A.phi0, A.dphi, A.inv_dphi, A.n, fa);
-                            const int ib = ring_slot(phi_w, Bn.phi0, Bn.dphi, Bn.inv_dphi, Bn.n, fb);
-                            const int l0 = fp_local(A, ia), l1 = fp_local(A, next_slot(ia, A.n));
-                            const int l2 = fp_local(Bn, ib), l3 = fp_local(Bn, next_slot(ib, Bn.n));
-                            if ((l0 | l1 | l2 | l3) >= 0) {
-                                const double wa = w_above * bv, wb = w_below * bv;
-                                atomicAdd(&W[l0], wa * (1.0 - fa));
-                                atomicAdd(&W[l1], wa * fa);
-                                atomicAdd(&W[l2], wb * (1.0 - fb));
-                                atomicAdd(&W[l3], wb * fb);
-                                ok = true;
-                            }
-                        }
-                    }
-                } else {
-                    const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-                    int ring, pos;
-                    nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
-                    const int lr = ring - ir_lo;
-                    const int li = (lr >= 0 && lr < nr) ? fp_local(rings[lr], pos) : -1;
-                    if (li >= 0) {
-                        atomicAdd(&W[li], bv);
-                        ok = true;
-                    }
-                }
-                if (!ok) slow |= 1ull << it;
-            }
-            // general code for the flagged beam pixels
-            while (slow) {
-                const int it = __builtin_ctzll(slow);
-                slow &= slow - 1;
-                const int64_t ss = s0 + (int64_t)it * kFpThreads + tid;
-                const double v0 = (double)a.vec_orig[3 * ss];
-                const double v1 = (double)a.vec_orig[3 * ss + 1];
-                const double v2 = (double)a.vec_orig[3 * ss + 2];
-                const double bv = (double)a.beam_vals[ss];
-                double vx, vy, vz;
-                rotate(q, v0, v1, v2, vx, vy, vz);
-                const double phi_w = wrap_phi(vy, vx);
-                if constexpr (MODE == 0) {
-                    const Acc4 d = fp_slow_bilinear<QU, CS>(&a, fmax(-1.0, fmin(1.0, vz)), phi_w, bv,
-                                                            rings, ir_lo, nr, W, q.z_pts,
-                                                            q.sth_pts, q.phi_pts);
-#pragma unroll
-                    for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
-                } else {
-                    const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-                    int ring, pos;
-                    nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
-                    const RingEntry& e = a.rings[ring];
-                    const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + pos, e.z,
-                                                      e.sth, e.phi0 + (double)pos * e.dphi, bv,
-                                                      q.z_pts, q.sth_pts, q.phi_pts);
-#pragma unroll
-                    for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
-                }
-            }
-        }
-        __syncthreads();
-
-        // ---- phase 2: footprint sweep, spin-2 + one gather per map pixel
-        double acc[kMaxSlots] = {accd[0], accd[1], accd[2], accd[3]};
-        int r = 0;
-        for (int j = tid; j < total; j += kFpThreads) {
-            while (r + 1 < nr && j >= rings[r].off + rings[r].width) ++r;
-            const double wsum = W[j];
-            if (wsum == 0.0) continue;
-            const FpRing& e = rings[r];
-            int pos = e.i0 + (j - e.off);
-            if (pos >= e.n) pos -= e.n;
-            const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
-            if constexpr (QU) {
-                const double phi_n = e.phi0 + (double)pos * e.dphi;
-                double c2d, s2d;
-                if (q.phi_pts == phi_n && q.sth_pts == e.sth && q.z_pts == e.z) {
-                    c2d = 1.0;
-                    s2d = 0.0;
-                } else {
-                    double h = (q.phi_pts - phi_n) * 0.5;
-                    if (h > 0.5 * kPi) h -= kPi;
-                    if (h < -0.5 * kPi) h += kPi;
-                    double sh, ch;
-                    sincos_half(h, sh, ch);
-                    spin2_from_half(spin2_ring(e.z, e.sth, q.z_pts, q.sth_pts), sh, ch, c2d, s2d);
-                }
-                const double qv = (double)m.x, uv = (double)m.y;
-                acc[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc[0]);
-                acc[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc[1]);
-#pragma unroll
-                for (int k = 2; k < CS; ++k)
-                    if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
-            } else {
-#pragma unroll
-                for (int k = 0; k < CS; ++k)
-                    if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
-            }
-        }
-
-        // ---- block reduction and write-out
-#pragma unroll
-        for (int k = 0; k < kMaxSlots; ++k) {
-            if (k < CS) {
-                const double v = wave_sum(acc[k]);
-                if (lane == 0) red[wave * kMaxSlots + k] = v;
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            // one thread, slots in order: two slots may share a row (c_q == c_u)
-            for (int k = 0; k < kMaxSlots && k < CS; ++k) {
-                if (k >= a.nslots) break;
-                double sum = 0.0;
-                for (int w = 0; w < kFpWaves; ++w) sum += red[w * kMaxSlots + k];
-                a.tod[(int64_t)a.row[k] * a.B + b] += sum;
-            }
-        }
-        __syncthreads();
+        if (lane == 0) ishared[0] = laid;  // grid cells [0, laid) belong to rings
     }
+    __syncthreads();
+    const int total = ishared[0];
+    for (int j = tid; j < total; j += kFpThreads) W[j] = 0.0;
+    __syncthreads();
+
+    // ---- phase 1: beam sweep, scatter weights into the LDS grid.
+    // A lean loop handles the common case (both rings laid out, all four
+    // neighbours inside the grid, no transcendental beyond atan2/sqrt);
+    // any beam pixel that needs more is flagged in a per-thread bit mask
+    // and redone by the general code in a second loop, so the hot loop's
+    // register allocation carries none of the rare paths.
+    double accd[kMaxSlots] = {0.0, 0.0, 0.0, 0.0};  // outside-grid sums
+    for (int64_t s0 = 0; s0 < a.S; s0 += (int64_t)kFpThreads * 64) {
+        uint64_t slow = 0;
+        float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
+        int64_t s = s0 + tid;
+        if (s < a.S) {
+            nv0 = a.vec_orig[3 * s];
+            nv1 = a.vec_orig[3 * s + 1];
+            nv2 = a.vec_orig[3 * s + 2];
+            nbv = a.beam_vals[s];
+        }
+        for (int it = 0; it < 64 && s < a.S; ++it, s += kFpThreads) {
+            const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
+            const double bv = (double)nbv;
+            const int64_t sn = s + kFpThreads;
+            if (it + 1 < 64 && sn < a.S) {  // fetch the next beam pixel early
+                nv0 = a.vec_orig[3 * sn];
+                nv1 = a.vec_orig[3 * sn + 1];
+                nv2 = a.vec_orig[3 * sn + 2];
+                nbv = a.beam_vals[sn];
+            }
+            double vx, vy, vz;
+            rotate(q, v0, v1, v2, vx, vy, vz);
+            const double phi_w = wrap_phi(vy, vx);
+            bool ok = false;
+            if constexpr (MODE == 0) {
+                const double z = fmax(-1.0, fmin(1.0, vz));
+                const int ira = ring_above(nside, z);
+                const int la = ira - ir_lo;
+                if (ira >= 1 && ira + 1 <= 4 * nside - 1 && la >= 0 && la + 1 < nr) {
+                    const FpRing& A = rings[la];
+                    const FpRing& Bn = rings[la + 1];
+                    const double sq = sqrt(fmax(0.0, (1.0 - z) * (1.0 + z)));
+                    const double x = fma(sq, A.z, -(z * A.sth));
+                    const double xa = phi_w - A.phi0, xb = phi_w - Bn.phi0;
+                    if (fabs(x) < 1e-2 && fabs(xa) >= 0x1p-960 && fabs(xb) >= 0x1p-960) {
+                        const double x2 = x * x;
+                        double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
+                        pp = fma(x2, pp, 3.0 / 40.0);
+                        pp = fma(x2, pp, 1.0 / 6.0);
+                        const double w_below = fma(x * x2, pp, x) * A.inv_dth;
+                        const double w_above = 1.0 - w_below;
+                        double fa, fb;
+                        const int ia = ring_slot(phi_w, A.phi0, A.dphi, A.inv_dphi, A.n, fa);
+                        const int ib = ring_slot(phi_w, Bn.phi0, Bn.dphi, Bn.inv_dphi, Bn.n, fb);
+                        const int l0 = fp_local(A, ia), l1 = fp_local(A, next_slot(ia, A.n));
+                        const int l2 = fp_local(Bn, ib), l3 = fp_local(Bn, next_slot(ib, Bn.n));
+                        if ((l0 | l1 | l2 | l3) >= 0) {
+                            const double wa = w_above * bv, wb = w_below * bv;
+                            atomicAdd(&W[l0], wa * (1.0 - fa));
+                            atomicAdd(&W[l1], wa * fa);
+                            atomicAdd(&W[l2], wb * (1.0 - fb));
+                            atomicAdd(&W[l3], wb * fb);
+                            ok = true;
+                        }
+                    }
+                }
+            } else {
+                const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
+                int ring, pos;
+                nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
+                const int lr = ring - ir_lo;
+                const int li = (lr >= 0 && lr < nr) ? fp_local(rings[lr], pos) : -1;
+                if (li >= 0) {
+                    atomicAdd(&W[li], bv);
+                    ok = true;
+                }
+            }
+            if (!ok) slow |= 1ull << it;
+        }
+        // general code for the flagged beam pixels
+        while (slow) {
+            const int it = __builtin_ctzll(slow);
+            slow &= slow - 1;
+            const int64_t ss = s0 + (int64_t)it * kFpThreads + tid;
+            const double v0 = (double)a.vec_orig[3 * ss];
+            const double v1 = (double)a.vec_orig[3 * ss + 1];
+            const double v2 = (double)a.vec_orig[3 * ss + 2];
+            const double bv = (double)a.beam_vals[ss];
+            double vx, vy, vz;
+            rotate(q, v0, v1, v2, vx, vy, vz);
+            const double phi_w = wrap_phi(vy, vx);
+            if constexpr (MODE == 0) {
+                const Acc4 d = fp_slow_bilinear<QU, CS>(&a, fmax(-1.0, fmin(1.0, vz)), phi_w, bv,
+                                                        rings, ir_lo, nr, W, q.z_pts,
+                                                        q.sth_pts, q.phi_pts);
+#pragma unroll
+                for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
+            } else {
+                const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
+                int ring, pos;
+                nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
+                const RingEntry& e = a.rings[ring];
+                const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + pos, e.z,
+                                                  e.sth, e.phi0 + (double)pos * e.dphi, bv,
+                                                  q.z_pts, q.sth_pts, q.phi_pts);
+#pragma unroll
+                for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: footprint sweep, spin-2 + one gather per map pixel
+    double acc[kMaxSlots] = {accd[0], accd[1], accd[2], accd[3]};
+    int r = 0;
+    for (int j = tid; j < total; j += kFpThreads) {
+        while (r + 1 < nr && j >= rings[r].off + rings[r].width) ++r;
+        const double wsum = W[j];
+        if (wsum == 0.0) continue;
+        const FpRing& e = rings[r];
+        int pos = e.i0 + (j - e.off);
+        if (pos >= e.n) pos -= e.n;
+        const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
+        if constexpr (QU) {
+            const double phi_n = e.phi0 + (double)pos * e.dphi;
+            double c2d, s2d;
+            if (q.phi_pts == phi_n && q.sth_pts == e.sth && q.z_pts == e.z) {
+                c2d = 1.0;
+                s2d = 0.0;
+            } else {
+                double h = (q.phi_pts - phi_n) * 0.5;
+                if (h > 0.5 * kPi) h -= kPi;
+                if (h < -0.5 * kPi) h += kPi;
+                double sh, ch;
+                sincos_half(h, sh, ch);
+                spin2_from_half(spin2_ring(e.z, e.sth, q.z_pts, q.sth_pts), sh, ch, c2d, s2d);
+            }
+            const double qv = (double)m.x, uv = (double)m.y;
+            acc[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc[0]);
+            acc[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc[1]);
+#pragma unroll
+            for (int k = 2; k < CS; ++k)
+                if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < CS; ++k)
+                if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
+        }
+    }
+
+    // ---- block reduction and write-out
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k) {
+        if (k < CS) {
+            const double v = wave_sum(acc[k]);
+            if (lane == 0) red[wave * kMaxSlots + k] = v;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // one thread, slots in order: two slots may share a row (c_q == c_u)
+        for (int k = 0; k < kMaxSlots && k < CS; ++k) {
+            if (k >= a.nslots) break;
+            double sum = 0.0;
+            for (int w = 0; w < kFpWaves; ++w) sum += red[w * kMaxSlots + k];
+            a.tod[(int64_t)a.row[k] * a.B + b] += sum;
+        }
+    }
+    __syncthreads();
+}
+
+template <int MODE, bool QU, int CS, int MINW>
+__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, const FpConf fc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const FpLds L = fp_lds(smem, fc);
+    // XCD-aware sample order: workgroups are dispatched round-robin over the
+    // 8 XCDs, so block bid runs on XCD bid % 8.  Each XCD gets a contiguous
+    // eighth of the samples, so concurrently processed samples on one XCD
+    // have overlapping footprints and share that XCD's L2.
+    const int G = gridDim.x;
+    const int xcd = blockIdx.x & 7;
+    const int per = G >> 3;
+    const int kx = blockIdx.x >> 3;
+    const int64_t chunk = (a.B + 7) / 8;
+    const int64_t bbeg = (int64_t)xcd * chunk;
+    const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
+    if (bbeg + kx >= bend) return;  // uniform over the block
+    const BeamFrame bf = fp_beam_frame(a, bbeg + kx, L);
+    for (int64_t b = bbeg + kx; b < bend; b += per) fp_sample<MODE, QU, CS>(a, b, bf, fc, L);
+}
+
+// ======================================================= batched kernel
+
+// Shared fields of a batch; per-detector fields come from todg_work_item.
+struct BatchArgs {
+    const todg_work_item* items;  // device array
+    int n_items;
+    int64_t total;                // sum of items[i].B
+    const float* map;
+    const RingEntry* rings;
+    int64_t npix;
+    int nside;
+    int nslots;
+    int row[kMaxSlots];
+    unsigned int* queue;          // 8 per-XCD claim counters, zeroed before launch
+    int* status;                  // per-item status words or null
+    int chunk;                    // samples per claim
+};
+
+__device__ __forceinline__ int find_item(const todg_work_item* items, int n, int64_t g) {
+    int lo = 0, hi = n - 1;  // last item with first <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].first <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool finite_sample(const TodArgs& a, int64_t b) {
+    const float v[10] = {a.axes[3 * b], a.axes[3 * b + 1], a.axes[3 * b + 2], a.cos_a[b], a.sin_a[b],
+                         a.ax_pts[3 * b], a.ax_pts[3 * b + 1], a.ax_pts[3 * b + 2], a.cos_p[b],
+                         a.sin_p[b]};
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) ok = ok && isfinite(v[k]);
+    return ok;
+}
+
+// Persistent multi-detector kernel.  The global sample range (all items'
+// samples back to back) is split into 8 contiguous parts, one work queue per
+// XCD; a workgroup claims `chunk` samples at a time from its own XCD's
+// queue and, once that is drained, from the others, so uneven beams and
+// chunk lengths balance themselves.  Every workgroup exits when all eight
+// queues are drained.  A sample with a non-finite pointing operand is
+// skipped and flags its item's status word.
+template <int MODE, bool QU, int CS, int MINW>
+__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchArgs ba,
+                                                                   const FpConf fc) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const FpLds L = fp_lds(smem, fc);
+    const int tid = threadIdx.x;
+    const int xcd = blockIdx.x & 7;
+    const int64_t qlen = (ba.total + 7) / 8;
+    int cur = -1;
+    TodArgs a;
+    a.map = ba.map;
+    a.rings = ba.rings;
+    a.npix = ba.npix;
+    a.nside = ba.nside;
+    a.nslots = ba.nslots;
+#pragma unroll
+    for (int k = 0; k < kMaxSlots; ++k) a.row[k] = ba.row[k];
+    BeamFrame bf = {0.0, 0.0, 1.0, 0.0, 1.0};
+    bool have_frame = false;
+    for (int att = 0; att < 8;) {
+        const int qi = (xcd + att) & 7;
+        const int64_t qbeg = (int64_t)qi * qlen;
+        const int64_t qend = (qbeg + qlen < ba.total) ? qbeg + qlen : ba.total;
+        if (tid == 0) L.ishared[1] = (int)atomicAdd(&ba.queue[qi], (unsigned int)ba.chunk);
+        __syncthreads();
+        const int64_t g0 = qbeg + (int64_t)(unsigned int)L.ishared[1];
+        __syncthreads();
+        if (g0 >= qend) {
+            ++att;
+            continue;
+        }
+        const int64_t g1 = (g0 + ba.chunk < qend) ? g0 + ba.chunk : qend;
+        for (int64_t g = g0; g < g1; ++g) {
+            const int it = find_item(ba.items, ba.n_items, g);
+            const todg_work_item& w = ba.items[it];
+            if (it != cur) {
+                a.vec_orig = w.vec_orig;
+                a.beam_vals = w.beam_vals;
+                a.axes = w.axes;
+                a.cos_a = w.cos_a;
+                a.sin_a = w.sin_a;
+                a.ax_pts = w.ax_pts;
+                a.cos_p = w.cos_p;
+                a.sin_p = w.sin_p;
+                a.tod = w.tod;
+                a.B = w.B;
+                a.S = w.S;
+                cur = it;
+                have_frame = false;
+            }
+            const int64_t b = g - w.first;
+            if (a.S <= 0) continue;
+            if (!finite_sample(a, b)) {
+                if (tid == 0 && ba.status) atomicOr(&ba.status[it], 1);
+                continue;
+            }
+            if (!have_frame) {  // beam frame from the item's first finite sample seen here
+                bf = fp_beam_frame(a, b, L);
+                have_frame = true;
+            }
+            fp_sample<MODE, QU, CS>(a, b, bf, fc, L);
+        }
+    }
+}
+
+__global__ void k_zero_u32(unsigned int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ================================================== index-path dump kernels
@@ -910,6 +1065,40 @@ hipError_t launch_fp_cs(const TodArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+template <int MODE, bool QU, int CS, int MINW>
+hipError_t launch_batch_cs(const BatchArgs& ba, hipStream_t st) {
+    const FpConf fc{kFpCap, kFpMaxRings};
+    const size_t smem = fp_smem_bytes(fc);
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute((const void*)k_tod_fp_batch<MODE, QU, CS, MINW>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    });
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ba.queue, 8);
+    int64_t blocks = (int64_t)cu_count() * (MINW >= 4 ? 2 : 1) * 2;
+    blocks = ((blocks + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_tod_fp_batch<MODE, QU, CS, MINW>), dim3((unsigned)blocks),
+                       dim3(kFpThreads), smem, st, ba, fc);
+    return hipGetLastError();
+}
+
+template <int MODE, bool QU, int CS>
+hipError_t launch_batch_v(const BatchArgs& ba, int variant, hipStream_t st) {
+    if (variant == 8) return launch_batch_cs<MODE, QU, CS, 4>(ba, st);
+    return launch_batch_cs<MODE, QU, CS, 2>(ba, st);
+}
+
+template <int MODE>
+hipError_t launch_batch_mode(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st) {
+    if (qu) {
+        if (CS == 2) return launch_batch_v<MODE, true, 2>(ba, variant, st);
+        return launch_batch_v<MODE, true, 4>(ba, variant, st);
+    }
+    if (CS == 1) return launch_batch_v<MODE, false, 1>(ba, variant, st);
+    if (CS == 2) return launch_batch_v<MODE, false, 2>(ba, variant, st);
+    return launch_batch_v<MODE, false, 4>(ba, variant, st);
+}
+
 template <int MODE, bool QU, int CS>
 hipError_t launch_cs(const TodArgs& a, int variant, hipStream_t st) {
     if (variant == 1 || variant == 2 || variant == 4)
@@ -1045,6 +1234,47 @@ int todg_gather_accum_nearest(const float* vec_orig, const float* axes, const fl
                               int c_q, int c_u, int variant, void* stream) {
     return run_tod(1, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
                    beam_vals, B, S, tod, c_q, c_u, variant, stream);
+}
+
+int todg_gather_accum_batch(int mode, const todg_work_item* items, int n_items,
+                            int64_t total_samples, int64_t nside, const float* map_packed, int C,
+                            int c_q, int c_u, int chunk, unsigned int* queue, int* item_status,
+                            int variant, void* stream) {
+    if (mode != 0 && mode != 1) return fail(TODG_ERR_ARG, "mode must be 0 (bilinear) or 1 (nearest)");
+    if (!is_pow2(nside) || nside > 8192)
+        return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
+    if (n_items < 0 || total_samples < 0) return fail(TODG_ERR_ARG, "negative sizes");
+    if (n_items == 0 || total_samples == 0) return TODG_OK;
+    if (!items || !map_packed || !queue) return fail(TODG_ERR_ARG, "null pointer argument");
+    // claim counters stay below 2^31: <= 2^30 samples per queue plus the
+    // overshoot of the final claims (blocks x chunk <= 2^12 x 2^16)
+    if (chunk < 1 || chunk > (1 << 16)) return fail(TODG_ERR_ARG, "chunk must be in [1, 65536]");
+    if (total_samples > ((int64_t)8 << 30)) return fail(TODG_ERR_ARG, "too many samples per launch");
+    bool qu;
+    int perm[kMaxSlots], nslots = 0;
+    BatchArgs ba;
+    int rc = slot_layout(C, c_q, c_u, &qu, perm, ba.row, &nslots);
+    if (rc) return rc;
+    const RingEntry* rings = nullptr;
+    rc = get_ring_table(nside, &rings);
+    if (rc) return rc;
+    ba.items = items;
+    ba.n_items = n_items;
+    ba.total = total_samples;
+    ba.map = map_packed;
+    ba.rings = rings;
+    ba.npix = 12 * nside * nside;
+    ba.nside = (int)nside;
+    ba.nslots = nslots;
+    ba.queue = queue;
+    ba.status = item_status;
+    ba.chunk = chunk;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int CS = slots_stride(nslots);
+    hipError_t e = mode == 0 ? launch_batch_mode<0>(ba, qu, CS, variant, st)
+                             : launch_batch_mode<1>(ba, qu, CS, variant, st);
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TODG_OK;
 }
 
 int todg_bilinear_indices(const float* vec_orig, const float* axes, const float* cos_a,

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import torch
 
-from . import kernels
+from . import autotune, kernels
 from .device import device, to_dev
 from .tod_rotations import _rotation_params
 
@@ -29,7 +29,7 @@ def _qu_slots(comp_indices):
 
 
 def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mode="bilinear",
-                   variant=kernels.DEFAULT_VARIANT):
+                   variant=None):
     """Beam-weighted sky samples of one beam entry for a batch of pointings.
 
     Args mirror tod_core.py:52-96: ``data`` needs ``vec_orig (S,3)``,
@@ -70,6 +70,9 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
         c_q = c_u = -1
         use_nearest = False
 
+    if variant is None:  # the autotuner's choice for this workload shape, if any
+        variant = autotune.tuned_variant(int(nside), S, "nearest" if use_nearest else "bilinear",
+                                         C, c_q >= 0 and c_u >= 0, kernels.DEFAULT_VARIANT)
     tod_arr = torch.zeros((C, B), dtype=torch.float64, device=device())
     fn = kernels._gather_accum_nearest_jit if use_nearest else kernels._gather_accum_fused_jit
     fn(vec_orig, *params, nside, mp_stacked, beam_vals, B, S, tod_arr, c_q, c_u, variant=variant)
