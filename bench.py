@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--mode", choices=["bilinear", "nearest"], default="bilinear")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -132,6 +134,12 @@ def main():
         if ev1 is not None:
             ev1.record()
 
+    if args.copy_calib:
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)  # 1 GiB read + 1 GiB write of known size
+        torch.cuda.synchronize()
+        del src, dst
     for i in range(args.warmup):
         step()
         torch.cuda.synchronize()

@@ -45,6 +45,10 @@ _SIGS = {
     "or_nearest_indices": (None, [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _p]),
     "or_rotated_dirs": (None, [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _p]),
     "or_max_threads": (ctypes.c_int, []),
+    "or_bicubic_weights": (_i64, [_i64, _f64, _f64, _p, _p]),
+    "or_gather_accum_bicubic": (
+        None, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, ctypes.c_int]),
+    "or_interp_values": (None, [_i64, _p, _i64, _i64, _p, _p, ctypes.c_int, _p]),
 }
 
 
@@ -249,6 +253,31 @@ def nearest_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, b
     lib().or_nearest_indices(_ptr(v), *[_ptr(o) for o in ops], int(nside), S, int(b0), int(nb),
                              _ptr(pix))
     return pix
+
+
+def bicubic_weights(nside, z, phi):
+    """Reconstructed bicubic judge: (stencil pixels, weights), README.md:120 +
+    numba_healpy.py:892-977 (parity unpinned: the reference module is absent)."""
+    p = np.zeros(64, np.int64)
+    w = np.zeros(64)
+    m = lib().or_bicubic_weights(nside, float(z), float(phi), _ptr(p), _ptr(w))
+    return p[:m].copy(), w[:m].copy()
+
+
+def gather_accum_bicubic(*args, **kw):
+    """Beam TOD with the bicubic judge (accumulates into tod)."""
+    return _kernel(lib().or_gather_accum_bicubic, *args, **kw)
+
+
+def interp_values(nside, mp, z, phi, mode):
+    """Interpolate one map field at directions (z, phi): mode 'bilinear' or 'bicubic'."""
+    m = _c(mp, np.float32)
+    zz = _c(z, np.float64)
+    pp = _c(phi, np.float64)
+    out = np.empty(zz.shape[0])
+    lib().or_interp_values(nside, _ptr(m), m.shape[0], zz.shape[0], _ptr(zz), _ptr(pp),
+                           0 if mode == "bilinear" else 1, _ptr(out))
+    return out
 
 
 def max_threads():
