@@ -404,16 +404,40 @@ __device__ __forceinline__ Acc4 direct_one(const float* __restrict__ map, int64_
 constexpr int kFpThreads = 512;
 constexpr int kFpWaves = kFpThreads / 64;
 
-// One footprint ring in LDS.
+// Certified fast index path (fp_phase1).  A ring or slot coordinate computed
+// there differs from the reference's binary64 value by far less than 1e-10
+// (see fp_phase1); its integer part is taken only when the coordinate is at
+// least kCertSlot (kCertPolar for the polar-cap ring formula, whose slope
+// grows near the pole) away from an integer, otherwise the beam pixel is
+// recomputed by the exact path.
+constexpr double kCertSlot = 1e-9;
+constexpr double kCertPolar = 1e-6;
+// Largest |tan| of the longitude offset from the footprint centre that the
+// fast path's atan series takes (truncation t^13/13 < 1e-21 at 0.03).
+constexpr double kMaxTan = 0.03;
+
+// One footprint ring in LDS.  The fast loop reads the first 48 bytes.
 struct alignas(16) FpRing {
-    double z, sth, theta, phi0, dphi, inv_dphi, inv_dth;
+    double base;      // slot coordinate of the footprint centre, window-relative
+    double inv_dphi;  // 1 / dphi
+    double z, sth, inv_dth;
+    int width, off;   // window width in slots, first weight-grid cell
+    double theta, phi0, dphi;
     int64_t first;
-    int n, i0, width, off;
+    int n, i0;        // ring pixels, first slot of the window
+};
+
+// Per-sample values shared by the workgroup (written by wave 0).
+struct alignas(16) FpFrame {
+    double m[9];   // m[3*i + j]: row i, column j of Rz(-phi_f) * R2 * R1
+    double q[12];  // Sample rotation fields: kx ky kz ca sa oma bx by bz cp sp omp
+    double z_pts, sth_pts, phi_pts;
+    int ir_lo, nr, total, pad;
 };
 
 struct FpConf {
     int cap;        // weight-grid entries
-    int max_rings;  // FpRing entries
+    int max_rings;  // FpRing entries per buffer
 };
 
 // Local index of (ring entry e, slot pos in [0, n]) in the weight grid, or -1.
@@ -425,12 +449,13 @@ __device__ __forceinline__ int fp_local(const FpRing& e, int pos) {
 }
 
 // Bilinear neighbours through the global ring table (pole caps, rings
-// outside the layout): LDS cell when laid out, direct contribution otherwise.
+// outside the layout, uncertified coordinates): LDS cell when laid out,
+// direct contribution otherwise.
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_slow_bilinear(const TodArgs* __restrict__ ap, double z,
-                                              double phi_w, double bv, const FpRing* rl,
-                                              int ir_lo, int nr, double* W, double z_pts,
-                                              double sth_pts, double phi_pts) {
+                                                 double phi_w, double bv, const FpRing* rl,
+                                                 int ir_lo, int nr, double* W, double z_pts,
+                                                 double sth_pts, double phi_pts) {
     const TodArgs& a = *ap;
     Nb o;
     ring_interp(a.nside, a.rings, z, phi_w, o);
@@ -451,20 +476,33 @@ __device__ __forceinline__ Acc4 fp_slow_bilinear(const TodArgs* __restrict__ ap,
     return r;
 }
 
-// LDS carve-up of the footprint kernels.
+// LDS carve-up of the footprint kernels.  Ring descriptors and frames are
+// double-buffered so that wave 0 can lay out sample b+1 while the other
+// waves still read sample b's.
 struct FpLds {
-    FpRing* rings;  // fc.max_rings ring descriptors
-    double* W;      // fc.cap weight-grid cells
-    double* red;    // kFpWaves * kMaxSlots reduction slots
-    int* ishared;   // 4 ints
+    FpRing* rings[2];  // fc.max_rings descriptors each
+    FpFrame* frame[2];
+    double* W;         // fc.cap weight-grid cells
+    double* red;       // kFpWaves * kMaxSlots reduction slots
+    double* scratch;   // kFpWaves doubles for fp_beam_frame
 };
 
 __device__ __forceinline__ FpLds fp_lds(unsigned char* smem, const FpConf& fc) {
     FpLds L;
-    L.rings = reinterpret_cast<FpRing*>(smem);
-    L.W = reinterpret_cast<double*>(smem + (size_t)fc.max_rings * sizeof(FpRing));
-    L.red = L.W + fc.cap;
-    L.ishared = reinterpret_cast<int*>(L.red + kFpWaves * kMaxSlots);
+    unsigned char* p = smem;
+    L.rings[0] = reinterpret_cast<FpRing*>(p);
+    p += (size_t)fc.max_rings * sizeof(FpRing);
+    L.rings[1] = reinterpret_cast<FpRing*>(p);
+    p += (size_t)fc.max_rings * sizeof(FpRing);
+    L.frame[0] = reinterpret_cast<FpFrame*>(p);
+    p += sizeof(FpFrame);
+    L.frame[1] = reinterpret_cast<FpFrame*>(p);
+    p += sizeof(FpFrame);
+    L.W = reinterpret_cast<double*>(p);
+    p += (size_t)fc.cap * sizeof(double);
+    L.red = reinterpret_cast<double*>(p);
+    p += (size_t)kFpWaves * kMaxSlots * sizeof(double);
+    L.scratch = reinterpret_cast<double*>(p);
     return L;
 }
 
@@ -482,7 +520,7 @@ __device__ __forceinline__ BeamFrame fp_beam_frame(const TodArgs& a, int64_t b0,
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    double* red = L.red;
+    double* red = L.scratch;
     double cx, cy, cz, cosr;
     {
         Sample q;
@@ -524,32 +562,28 @@ __device__ __forceinline__ BeamFrame fp_beam_frame(const TodArgs& a, int64_t b0,
     return f;
 }
 
-// All threads of the block: one sample's footprint layout, beam sweep,
-// footprint sweep and tod update.
-template <int MODE, bool QU, int CS>
-__device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const BeamFrame& bf,
-                                          const FpConf& fc, const FpLds& L) {
-    const int tid = threadIdx.x;
-    const int lane = tid & 63;
-    const int wave = tid >> 6;
+// Wave 0 only: sample b's frame and footprint layout -- every ring and ring
+// slot that the rotated beam's bilinear neighbours can touch, as per-ring
+// slot windows packed into the weight grid.
+__device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const BeamFrame& bf,
+                                          const FpConf& fc, FpFrame* fr, FpRing* rings) {
+    const int lane = threadIdx.x & 63;
     const int nside = a.nside;
-    FpRing* rings = L.rings;
-    double* W = L.W;
-    double* red = L.red;
-    int* ishared = L.ishared;
-    const double cx = bf.cx, cy = bf.cy, cz = bf.cz, R = bf.R, cosR = bf.cosR;
     Sample q;
     load_sample(a, b, q);
     make_uniform(q);
     double fx, fy, fz;  // footprint centre = this sample's image of c
-    rotate(q, cx, cy, cz, fx, fy, fz);
+    rotate(q, bf.cx, bf.cy, bf.cz, fx, fy, fz);
     const double fzc = fmax(-1.0, fmin(1.0, fz));
     const double fth = acos(fzc);
     const double fst = sqrt(fmax(0.0, (1.0 - fzc) * (1.0 + fzc)));
     double fph = atan2(fy, fx);
     if (fph < 0.0) fph += kTwoPi;
+    const double rho = sqrt(fx * fx + fy * fy);
+    const double cf = rho > 0.0 ? fx / rho : 1.0;
+    const double sf = rho > 0.0 ? fy / rho : 0.0;
 
-    const double th_lo = fth - R, th_hi = fth + R;
+    const double th_lo = fth - bf.R, th_hi = fth + bf.R;
     int ir_lo = th_lo <= 0.0 ? 1 : ring_above(nside, cos(th_lo));
     int ir_hi = th_hi >= kPi ? 4 * nside - 1 : ring_above(nside, cos(th_hi)) + 1;
     ir_lo = ir_lo < 1 ? 1 : ir_lo;
@@ -557,75 +591,227 @@ __device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const Bea
     int nr = ir_hi - ir_lo + 1;
     if (nr > fc.max_rings || nr < 0) nr = 0;  // no layout: everything goes direct
 
-    // ---- per-ring slot windows (wave 0) and their prefix offsets
-    if (wave == 0) {
-        int carry = 0, laid = 0;
-        for (int r0 = 0; r0 < nr; r0 += 64) {
-            const int r = r0 + lane;
-            int width = 0, i0 = 0;
-            FpRing e;
-            if (r < nr) {
-                const RingEntry& g = a.rings[ir_lo + r];
-                e.z = g.z;
-                e.sth = g.sth;
-                e.theta = g.theta;
-                e.phi0 = g.phi0;
-                e.dphi = g.dphi;
-                e.inv_dphi = g.inv_dphi;
-                e.inv_dth = g.inv_dth;
-                e.first = g.first;
-                e.n = g.n;
-                const double den = fst * g.sth;
-                const double cd = den > 1e-12 ? (cosR - fzc * g.z) / den : -2.0;
-                if (cd <= -1.0) {
-                    width = g.n;
-                } else if (cd < 1.0) {
-                    const double dp = acos(cd);
-                    const double wf = 2.0 * dp / g.dphi + 6.0;
-                    if (wf >= (double)g.n) {
-                        width = g.n;
-                    } else {
-                        width = (int)wf;
-                        int f = (int)floor((fph - dp - g.phi0) / g.dphi) - 2;
-                        f %= g.n;
-                        i0 = f < 0 ? f + g.n : f;
-                    }
+    // Composite rotation: column j of R2 R1 is the reference rotation of the
+    // unit vector e_j; the rows are then turned by -phi_f about z, so that
+    // the rotated direction's longitude is phi_f + atan2(row1.v, row0.v).
+    if (lane < 3) {
+        double mx, my, mz;
+        rotate(q, lane == 0 ? 1.0 : 0.0, lane == 1 ? 1.0 : 0.0, lane == 2 ? 1.0 : 0.0, mx, my,
+               mz);
+        fr->m[lane] = fma(cf, mx, sf * my);
+        fr->m[3 + lane] = fma(cf, my, -(sf * mx));
+        fr->m[6 + lane] = mz;
+    }
+
+    int carry = 0, laid = 0;
+    for (int r0 = 0; r0 < nr; r0 += 64) {
+        const int r = r0 + lane;
+        int width = 0, i0 = 0;
+        double base = 0.0;
+        FpRing e;
+        if (r < nr) {
+            const RingEntry& g = a.rings[ir_lo + r];
+            e.inv_dphi = g.inv_dphi;
+            e.z = g.z;
+            e.sth = g.sth;
+            e.inv_dth = g.inv_dth;
+            e.theta = g.theta;
+            e.phi0 = g.phi0;
+            e.dphi = g.dphi;
+            e.first = g.first;
+            e.n = g.n;
+            const double tc = (fph - g.phi0) * g.inv_dphi;  // centre's slot coordinate
+            const double den = fst * g.sth;
+            const double cd = den > 1e-12 ? (bf.cosR - fzc * g.z) / den : -2.0;
+            bool full = false;
+            if (cd <= -1.0) {
+                full = true;
+            } else if (cd < 1.0) {
+                const double dp = acos(cd);
+                const double wf = 2.0 * dp / g.dphi + 6.0;
+                if (wf >= (double)g.n) {
+                    full = true;
+                } else {
+                    width = (int)wf;
+                    const int f = (int)floor((fph - dp - g.phi0) / g.dphi) - 2;
+                    base = tc - (double)f;
+                    const int fm = f % g.n;
+                    i0 = fm < 0 ? fm + g.n : fm;
                 }
             }
-            int incl = width;  // inclusive scan of widths across the wave
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const int t = __shfl_up(incl, d, 64);
-                if (lane >= d) incl += t;
+            if (full) {
+                width = g.n;
+                i0 = 0;
+                base = tc >= (double)g.n ? tc - (double)g.n : (tc < 0.0 ? tc + (double)g.n : tc);
             }
-            const int off = carry + incl - width;
-            const bool fits = off + width <= fc.cap;
-            int end = (r < nr && fits) ? off + width : 0;
-            if (r < nr) {
-                e.i0 = i0;
-                e.width = fits ? width : 0;  // beyond capacity: accumulated directly
-                e.off = fits ? off : 0;
-                rings[r] = e;
-            }
-#pragma unroll
-            for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
-            laid = max(laid, end);
-            carry += __shfl(incl, 63, 64);
         }
-        if (lane == 0) ishared[0] = laid;  // grid cells [0, laid) belong to rings
+        int incl = width;  // inclusive scan of widths across the wave
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        const int off = carry + incl - width;
+        const bool fits = off + width <= fc.cap;
+        int end = (r < nr && fits) ? off + width : 0;
+        if (r < nr) {
+            e.base = base;
+            e.i0 = i0;
+            e.width = fits ? width : 0;  // beyond capacity: accumulated directly
+            e.off = fits ? off : 0;
+            rings[r] = e;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
+        laid = max(laid, end);
+        carry += __shfl(incl, 63, 64);
     }
-    __syncthreads();
-    const int total = ishared[0];
-    for (int j = tid; j < total; j += kFpThreads) W[j] = 0.0;
-    __syncthreads();
+    if (lane == 0) {
+        fr->q[0] = q.kx;
+        fr->q[1] = q.ky;
+        fr->q[2] = q.kz;
+        fr->q[3] = q.ca;
+        fr->q[4] = q.sa;
+        fr->q[5] = q.oma;
+        fr->q[6] = q.bx;
+        fr->q[7] = q.by;
+        fr->q[8] = q.bz;
+        fr->q[9] = q.cp;
+        fr->q[10] = q.sp;
+        fr->q[11] = q.omp;
+        fr->z_pts = q.z_pts;
+        fr->sth_pts = q.sth_pts;
+        fr->phi_pts = q.phi_pts;
+        fr->ir_lo = ir_lo;
+        fr->nr = nr;
+        fr->total = laid;  // grid cells [0, laid) belong to rings
+    }
+}
 
-    // ---- phase 1: beam sweep, scatter weights into the LDS grid.
-    // A lean loop handles the common case (both rings laid out, all four
-    // neighbours inside the grid, no transcendental beyond atan2/sqrt);
-    // any beam pixel that needs more is flagged in a per-thread bit mask
-    // and redone by the general code in a second loop, so the hot loop's
-    // register allocation carries none of the rare paths.
-    double accd[kMaxSlots] = {0.0, 0.0, 0.0, 0.0};  // outside-grid sums
+__device__ __forceinline__ void frame_sample(const FpFrame* fr, Sample& q) {
+    q.kx = fr->q[0];
+    q.ky = fr->q[1];
+    q.kz = fr->q[2];
+    q.ca = fr->q[3];
+    q.sa = fr->q[4];
+    q.oma = fr->q[5];
+    q.bx = fr->q[6];
+    q.by = fr->q[7];
+    q.bz = fr->q[8];
+    q.cp = fr->q[9];
+    q.sp = fr->q[10];
+    q.omp = fr->q[11];
+    q.z_pts = fr->z_pts;
+    q.sth_pts = fr->sth_pts;
+    q.phi_pts = fr->phi_pts;
+    make_uniform(q);
+}
+
+// sqrt(x) for x > 0 from the hardware reciprocal square root and one
+// Goldschmidt step (relative error ~1e-14).  Weights only.
+__device__ __forceinline__ double sqrt_fast(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    const double s = x * r;
+    const double h = 0.5 * r;
+    const double e = fma(-s, h, 0.5);
+    return fma(s, e, s);
+}
+
+// Bilinear fast path for one beam pixel (unit vector u, beam value bv):
+// returns false, touching nothing, when the exact path must take it.
+//
+// Index certification.  The rotated direction comes from the composite
+// matrix instead of two sequential Rodrigues steps, and its longitude as
+// phi_f + atan(t) instead of atan2; both differ from the reference's binary64
+// values by a few ulp (|dz| < 1e-15, |dphi| < 1e-15 for sin(theta) > 0.05).
+// A ring coordinate nside*(2 - 1.5 z) then differs by < 1e-11, a polar one
+// nside*sqrt(3(1-|z|)) by < 1e-7, and a slot coordinate (phi - phi0)/dphi by
+// < 1e-11 (including the reference's own rounding of its division), so a
+// coordinate whose fractional part lies in [eps, 1 - eps] has the
+// reference's integer part: ring_above, the two ring slots and hence all four
+// pixels are the reference's.  Everything else -- margins, pole caps, rings
+// or slots outside the laid-out windows, large longitude offsets -- goes to
+// the exact path.  Weights only need tolerance accuracy.
+__device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, int nside,
+                                                 int ir_lo, int nr, const FpRing* rings,
+                                                 double* W, double u0, double u1, double u2,
+                                                 double bv) {
+    const double xp = fma(m[0], u0, fma(m[1], u1, m[2] * u2));
+    const double yp = fma(m[3], u0, fma(m[4], u1, m[5] * u2));
+    const double zr = fma(m[6], u0, fma(m[7], u1, m[8] * u2));
+    const double z = fmax(-1.0, fmin(1.0, zr));
+    const double az = fabs(z);
+    double tr;
+    int ira;
+    bool ok;
+    if (az <= kTwoThirds) {
+        tr = fma(z, -1.5 * (double)nside, 2.0 * (double)nside);
+        ira = (int)tr;
+        const double fr = tr - (double)ira;
+        ok = fr > kCertSlot && fr < 1.0 - kCertSlot;
+    } else {
+        tr = (double)nside * sqrt_fast(3.0 * (1.0 - az));
+        const int it = (int)tr;
+        ira = z < 0.0 ? 4 * nside - it - 1 : it;
+        const double fr = tr - (double)it;
+        ok = fr > kCertPolar && fr < 1.0 - kCertPolar;
+    }
+    ok = ok && fabs(az - kTwoThirds) > 1e-12;
+    const int la = ira - ir_lo;
+    ok = ok && ira >= 1 && ira <= 4 * nside - 2 && la >= 0 && la + 1 < nr;
+    ok = ok && xp > 0.0 && fabs(yp) < kMaxTan * xp;
+    if (!ok) return false;
+    const FpRing& A = rings[la];
+    const FpRing& Bn = rings[la + 1];
+    // longitude offset from the footprint centre: atan(t), |t| < 0.03
+    const double t = yp * rcp_nr(xp);
+    const double t2 = t * t;
+    double p = fma(t2, -1.0 / 11.0, 1.0 / 9.0);
+    p = fma(t2, p, -1.0 / 7.0);
+    p = fma(t2, p, 1.0 / 5.0);
+    p = fma(t2, p, -1.0 / 3.0);
+    const double dl = fma(t * t2, p, t);
+    const double ta = fma(dl, A.inv_dphi, A.base);
+    const double tb = fma(dl, Bn.inv_dphi, Bn.base);
+    const int ia = (int)ta, ib = (int)tb;
+    const double fa = ta - (double)ia, fb = tb - (double)ib;
+    ok = ta >= 0.0 && tb >= 0.0 && ia + 1 < A.width && ib + 1 < Bn.width;
+    ok = ok && fa > kCertSlot && fa < 1.0 - kCertSlot && fb > kCertSlot && fb < 1.0 - kCertSlot;
+    // theta - theta_above = asin(sin(theta) z_a - z sin(theta_a))
+    const double sq = sqrt_fast((1.0 - z) * (1.0 + z));
+    const double x = fma(sq, A.z, -(z * A.sth));
+    ok = ok && fabs(x) < 1e-2;
+    if (!ok) return false;
+    const double x2 = x * x;
+    double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
+    pp = fma(x2, pp, 3.0 / 40.0);
+    pp = fma(x2, pp, 1.0 / 6.0);
+    const double wb = fma(x * x2, pp, x) * A.inv_dth * bv;
+    const double wa = bv - wb;
+    double* ca = W + A.off + ia;
+    double* cb = W + Bn.off + ib;
+    atomicAdd(ca, wa - wa * fa);
+    atomicAdd(ca + 1, wa * fa);
+    atomicAdd(cb, wb - wb * fb);
+    atomicAdd(cb + 1, wb * fb);
+    return true;
+}
+
+// All threads: phase 1 of one sample -- sweep the beam and scatter
+// beam_val * weight into the LDS weight grid; returns the outside-grid sums.
+template <int MODE, bool QU, int CS>
+__device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpFrame* fr,
+                                          const FpRing* rings, double* W) {
+    const int tid = threadIdx.x;
+    const int nside = a.nside;
+    const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
+    const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
+    double m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
+    Sample qn;  // nearest mode rotates with the exact path
+    if constexpr (MODE != 0) frame_sample(fr, qn);
+    Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};  // outside-grid sums
     for (int64_t s0 = 0; s0 < a.S; s0 += (int64_t)kFpThreads * 64) {
         uint64_t slow = 0;
         float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
@@ -646,43 +832,13 @@ __device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const Bea
                 nv2 = a.vec_orig[3 * sn + 2];
                 nbv = a.beam_vals[sn];
             }
-            double vx, vy, vz;
-            rotate(q, v0, v1, v2, vx, vy, vz);
-            const double phi_w = wrap_phi(vy, vx);
             bool ok = false;
             if constexpr (MODE == 0) {
-                const double z = fmax(-1.0, fmin(1.0, vz));
-                const int ira = ring_above(nside, z);
-                const int la = ira - ir_lo;
-                if (ira >= 1 && ira + 1 <= 4 * nside - 1 && la >= 0 && la + 1 < nr) {
-                    const FpRing& A = rings[la];
-                    const FpRing& Bn = rings[la + 1];
-                    const double sq = sqrt(fmax(0.0, (1.0 - z) * (1.0 + z)));
-                    const double x = fma(sq, A.z, -(z * A.sth));
-                    const double xa = phi_w - A.phi0, xb = phi_w - Bn.phi0;
-                    if (fabs(x) < 1e-2 && fabs(xa) >= 0x1p-960 && fabs(xb) >= 0x1p-960) {
-                        const double x2 = x * x;
-                        double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
-                        pp = fma(x2, pp, 3.0 / 40.0);
-                        pp = fma(x2, pp, 1.0 / 6.0);
-                        const double w_below = fma(x * x2, pp, x) * A.inv_dth;
-                        const double w_above = 1.0 - w_below;
-                        double fa, fb;
-                        const int ia = ring_slot(phi_w, A.phi0, A.dphi, A.inv_dphi, A.n, fa);
-                        const int ib = ring_slot(phi_w, Bn.phi0, Bn.dphi, Bn.inv_dphi, Bn.n, fb);
-                        const int l0 = fp_local(A, ia), l1 = fp_local(A, next_slot(ia, A.n));
-                        const int l2 = fp_local(Bn, ib), l3 = fp_local(Bn, next_slot(ib, Bn.n));
-                        if ((l0 | l1 | l2 | l3) >= 0) {
-                            const double wa = w_above * bv, wb = w_below * bv;
-                            atomicAdd(&W[l0], wa * (1.0 - fa));
-                            atomicAdd(&W[l1], wa * fa);
-                            atomicAdd(&W[l2], wb * (1.0 - fb));
-                            atomicAdd(&W[l3], wb * fb);
-                            ok = true;
-                        }
-                    }
-                }
+                ok = fp_fast_bilinear(m, nside, ir_lo, nr, rings, W, v0, v1, v2, bv);
             } else {
+                double vx, vy, vz;
+                rotate(qn, v0, v1, v2, vx, vy, vz);
+                const double phi_w = wrap_phi(vy, vx);
                 const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
                 int ring, pos;
                 nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
@@ -695,95 +851,152 @@ __device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const Bea
             }
             if (!ok) slow |= 1ull << it;
         }
-        // general code for the flagged beam pixels
-        while (slow) {
-            const int it = __builtin_ctzll(slow);
-            slow &= slow - 1;
-            const int64_t ss = s0 + (int64_t)it * kFpThreads + tid;
-            const double v0 = (double)a.vec_orig[3 * ss];
-            const double v1 = (double)a.vec_orig[3 * ss + 1];
-            const double v2 = (double)a.vec_orig[3 * ss + 2];
-            const double bv = (double)a.beam_vals[ss];
-            double vx, vy, vz;
-            rotate(q, v0, v1, v2, vx, vy, vz);
-            const double phi_w = wrap_phi(vy, vx);
-            if constexpr (MODE == 0) {
-                const Acc4 d = fp_slow_bilinear<QU, CS>(&a, fmax(-1.0, fmin(1.0, vz)), phi_w, bv,
-                                                        rings, ir_lo, nr, W, q.z_pts,
-                                                        q.sth_pts, q.phi_pts);
+        // exact path for the flagged beam pixels
+        if (slow) {
+            Sample q;
+            frame_sample(fr, q);
+            while (slow) {
+                const int it = __builtin_ctzll(slow);
+                slow &= slow - 1;
+                const int64_t ss = s0 + (int64_t)it * kFpThreads + tid;
+                const double v0 = (double)a.vec_orig[3 * ss];
+                const double v1 = (double)a.vec_orig[3 * ss + 1];
+                const double v2 = (double)a.vec_orig[3 * ss + 2];
+                const double bv = (double)a.beam_vals[ss];
+                double vx, vy, vz;
+                rotate(q, v0, v1, v2, vx, vy, vz);
+                const double phi_w = wrap_phi(vy, vx);
+                if constexpr (MODE == 0) {
+                    const Acc4 d = fp_slow_bilinear<QU, CS>(&a, fmax(-1.0, fmin(1.0, vz)), phi_w,
+                                                            bv, rings, ir_lo, nr, W, q.z_pts,
+                                                            q.sth_pts, q.phi_pts);
 #pragma unroll
-                for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
-            } else {
-                const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-                int ring, pos;
-                nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
-                const RingEntry& e = a.rings[ring];
-                const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + pos, e.z,
-                                                  e.sth, e.phi0 + (double)pos * e.dphi, bv,
-                                                  q.z_pts, q.sth_pts, q.phi_pts);
+                    for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
+                } else {
+                    const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
+                    int ring, pos;
+                    nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
+                    const RingEntry& e = a.rings[ring];
+                    const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + pos, e.z,
+                                                      e.sth, e.phi0 + (double)pos * e.dphi, bv,
+                                                      q.z_pts, q.sth_pts, q.phi_pts);
 #pragma unroll
-                for (int j = 0; j < kMaxSlots; ++j) accd[j] += d.v[j];
+                    for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
+                }
             }
         }
     }
-    __syncthreads();
+    return accd;
+}
 
-    // ---- phase 2: footprint sweep, spin-2 + one gather per map pixel
-    double acc[kMaxSlots] = {accd[0], accd[1], accd[2], accd[3]};
+// All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
+// gather per map pixel with nonzero weight; the cell is cleared for the next
+// sample.  Returns the per-thread sums (outside-grid sums included).
+template <bool QU, int CS>
+__device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr,
+                                          const FpRing* rings, double* W, Acc4 acc) {
+    const int tid = threadIdx.x;
+    const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
+    const int total = __builtin_amdgcn_readfirstlane(fr->total);
+    const double z_pts = uniform(fr->z_pts), sth_pts = uniform(fr->sth_pts);
+    const double phi_pts = uniform(fr->phi_pts);
     int r = 0;
     for (int j = tid; j < total; j += kFpThreads) {
         while (r + 1 < nr && j >= rings[r].off + rings[r].width) ++r;
         const double wsum = W[j];
         if (wsum == 0.0) continue;
+        W[j] = 0.0;
         const FpRing& e = rings[r];
         int pos = e.i0 + (j - e.off);
         if (pos >= e.n) pos -= e.n;
-        const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
+        const float4 mv = load_px<CS>(a.map, e.first + pos, a.npix);
         if constexpr (QU) {
             const double phi_n = e.phi0 + (double)pos * e.dphi;
             double c2d, s2d;
-            if (q.phi_pts == phi_n && q.sth_pts == e.sth && q.z_pts == e.z) {
+            if (phi_pts == phi_n && sth_pts == e.sth && z_pts == e.z) {
                 c2d = 1.0;
                 s2d = 0.0;
             } else {
-                double h = (q.phi_pts - phi_n) * 0.5;
+                double h = (phi_pts - phi_n) * 0.5;
                 if (h > 0.5 * kPi) h -= kPi;
                 if (h < -0.5 * kPi) h += kPi;
                 double sh, ch;
                 sincos_half(h, sh, ch);
-                spin2_from_half(spin2_ring(e.z, e.sth, q.z_pts, q.sth_pts), sh, ch, c2d, s2d);
+                spin2_from_half(spin2_ring(e.z, e.sth, z_pts, sth_pts), sh, ch, c2d, s2d);
             }
-            const double qv = (double)m.x, uv = (double)m.y;
-            acc[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc[0]);
-            acc[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc[1]);
+            const double qv = (double)mv.x, uv = (double)mv.y;
+            acc.v[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc.v[0]);
+            acc.v[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
 #pragma unroll
             for (int k = 2; k < CS; ++k)
-                if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
+                if (k < a.nslots) acc.v[k] = fma(wsum, (double)slot(mv, k), acc.v[k]);
         } else {
 #pragma unroll
             for (int k = 0; k < CS; ++k)
-                if (k < a.nslots) acc[k] = fma(wsum, (double)slot(m, k), acc[k]);
+                if (k < a.nslots) acc.v[k] = fma(wsum, (double)slot(mv, k), acc.v[k]);
         }
     }
+    return acc;
+}
 
-    // ---- block reduction and write-out
+// A sample whose block sums sit in red[] until thread 0 writes them.
+struct FpPending {
+    double* tod;
+    int64_t B, b;
+};
+
+// Thread 0: add the pending sample's block sums to its tod entries, slots in
+// order (two slots may share a row when c_q == c_u).  Callers guarantee that
+// every wave's red[] entries are complete (a barrier since they were made).
+__device__ __forceinline__ void fp_flush(const TodArgs& a, const FpLds& L, FpPending& pend) {
+    if (threadIdx.x == 0 && pend.tod) {
+        for (int k = 0; k < kMaxSlots && k < a.nslots; ++k) {
+            double sum = 0.0;
+            for (int w = 0; w < kFpWaves; ++w) sum += L.red[w * kMaxSlots + k];
+            pend.tod[(int64_t)a.row[k] * pend.B + pend.b] += sum;
+        }
+    }
+    pend.tod = nullptr;
+}
+
+// All threads of the block: one sample through layout, both phases and the
+// block reduction; two barriers per sample.  The tod entries of sample b are
+// written after the next sample's first barrier (or by fp_drain).
+template <int MODE, bool QU, int CS>
+__device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const BeamFrame& bf,
+                                          const FpConf& fc, const FpLds& L, int& buf,
+                                          FpPending& pend) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    FpFrame* fr = L.frame[buf];
+    FpRing* rings = L.rings[buf];
+    if (wave == 0) fp_layout(a, b, bf, fc, fr, rings);
+    __syncthreads();
+    fp_flush(a, L, pend);
+    Acc4 acc = fp_phase1<MODE, QU, CS>(a, fr, rings, L.W);
+    __syncthreads();
+    acc = fp_phase2<QU, CS>(a, fr, rings, L.W, acc);
 #pragma unroll
     for (int k = 0; k < kMaxSlots; ++k) {
         if (k < CS) {
-            const double v = wave_sum(acc[k]);
-            if (lane == 0) red[wave * kMaxSlots + k] = v;
+            const double v = wave_sum(acc.v[k]);
+            if (lane == 0) L.red[wave * kMaxSlots + k] = v;
         }
     }
+    pend.tod = a.tod;
+    pend.B = a.B;
+    pend.b = b;
+    buf ^= 1;
+}
+
+__device__ __forceinline__ void fp_drain(const TodArgs& a, const FpLds& L, FpPending& pend) {
     __syncthreads();
-    if (tid == 0) {
-        // one thread, slots in order: two slots may share a row (c_q == c_u)
-        for (int k = 0; k < kMaxSlots && k < CS; ++k) {
-            if (k >= a.nslots) break;
-            double sum = 0.0;
-            for (int w = 0; w < kFpWaves; ++w) sum += red[w * kMaxSlots + k];
-            a.tod[(int64_t)a.row[k] * a.B + b] += sum;
-        }
-    }
+    fp_flush(a, L, pend);
+    __syncthreads();
+}
+
+__device__ __forceinline__ void fp_clear_grid(const FpConf& fc, const FpLds& L) {
+    for (int j = threadIdx.x; j < fc.cap; j += kFpThreads) L.W[j] = 0.0;
     __syncthreads();
 }
 
@@ -803,8 +1016,13 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, co
     const int64_t bbeg = (int64_t)xcd * chunk;
     const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
     if (bbeg + kx >= bend) return;  // uniform over the block
+    fp_clear_grid(fc, L);
     const BeamFrame bf = fp_beam_frame(a, bbeg + kx, L);
-    for (int64_t b = bbeg + kx; b < bend; b += per) fp_sample<MODE, QU, CS>(a, b, bf, fc, L);
+    int buf = 0;
+    FpPending pend = {nullptr, 0, 0};
+    for (int64_t b = bbeg + kx; b < bend; b += per)
+        fp_sample<MODE, QU, CS>(a, b, bf, fc, L, buf, pend);
+    fp_drain(a, L, pend);
 }
 
 // ======================================================= batched kernel
@@ -873,13 +1091,17 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
     for (int k = 0; k < kMaxSlots; ++k) a.row[k] = ba.row[k];
     BeamFrame bf = {0.0, 0.0, 1.0, 0.0, 1.0};
     bool have_frame = false;
+    int buf = 0;
+    FpPending pend = {nullptr, 0, 0};
+    fp_clear_grid(fc, L);
+    int* claim = reinterpret_cast<int*>(L.scratch + kFpWaves);
     for (int att = 0; att < 8;) {
         const int qi = (xcd + att) & 7;
         const int64_t qbeg = (int64_t)qi * qlen;
         const int64_t qend = (qbeg + qlen < ba.total) ? qbeg + qlen : ba.total;
-        if (tid == 0) L.ishared[1] = (int)atomicAdd(&ba.queue[qi], (unsigned int)ba.chunk);
+        if (tid == 0) *claim = (int)atomicAdd(&ba.queue[qi], (unsigned int)ba.chunk);
         __syncthreads();
-        const int64_t g0 = qbeg + (int64_t)(unsigned int)L.ishared[1];
+        const int64_t g0 = qbeg + (int64_t)(unsigned int)*claim;
         __syncthreads();
         if (g0 >= qend) {
             ++att;
@@ -914,9 +1136,10 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
                 bf = fp_beam_frame(a, b, L);
                 have_frame = true;
             }
-            fp_sample<MODE, QU, CS>(a, b, bf, fc, L);
+            fp_sample<MODE, QU, CS>(a, b, bf, fc, L, buf, pend);
         }
     }
+    fp_drain(a, L, pend);
 }
 
 __global__ void k_zero_u32(unsigned int* p, int n) {
@@ -1019,14 +1242,16 @@ int slot_layout(int C, int c_q, int c_u, bool* qu, int perm[kMaxSlots], int row[
     return TODG_OK;
 }
 
-// Footprint-kernel LDS configuration: rings and weight-grid entries
-// (~53 KB per workgroup: two workgroups of 8 waves per CU).
-constexpr int kFpMaxRings = 192;
+// Footprint-kernel LDS configuration: two buffers of ring descriptors and
+// frames, the weight grid, reduction and scratch slots (~62 KB per
+// workgroup: two workgroups of 8 waves per CU).
+constexpr int kFpMaxRings = 128;
 constexpr int kFpCap = 4608;
 
 size_t fp_smem_bytes(const FpConf& fc) {
-    return (size_t)fc.max_rings * sizeof(FpRing) + (size_t)fc.cap * sizeof(double) +
-           (size_t)kFpWaves * kMaxSlots * sizeof(double) + 16;
+    return 2 * (size_t)fc.max_rings * sizeof(FpRing) + 2 * sizeof(FpFrame) +
+           (size_t)fc.cap * sizeof(double) + (size_t)kFpWaves * kMaxSlots * sizeof(double) +
+           (size_t)(kFpWaves + 2) * sizeof(double);
 }
 
 template <int MODE, bool QU, int CS>
