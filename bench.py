@@ -59,27 +59,33 @@ def cpu_baseline(vec, bv, params, mp_host, nside, B, S, target_s):
     import oracle as O
 
     cores = len(os.sched_getaffinity(0))
-    # calibrate on a small prefix, then run ~target_s worth of samples
-    nb = 64
-    sub = tuple(p[:nb] for p in params)
-    tod = np.zeros((3, nb))
-    t = time.perf_counter()
-    O.gather_accum_fused(vec, *sub, nside, mp_host, bv, nb, S, tod, 1, 2, nthreads=cores)
-    dt = time.perf_counter() - t
-    rate = nb * S / max(dt, 1e-6)
-    nb = int(min(B, max(nb, rate * target_s / S)))
-    sub = tuple(p[:nb] for p in params)
-    tod = np.zeros((3, nb))
-    t = time.perf_counter()
-    O.gather_accum_fused(vec, *sub, nside, mp_host, bv, nb, S, tod, 1, 2, nthreads=cores)
-    dt = time.perf_counter() - t
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        cores = min(cores, int(omp))  # this process's CPU share
+
+    def run(nb):
+        sub = tuple(p[:nb] for p in params)
+        tod = np.zeros((3, nb))
+        t = time.perf_counter()
+        O.gather_accum_fused(vec, *sub, nside, mp_host, bv, nb, S, tod, 1, 2, nthreads=cores)
+        return time.perf_counter() - t
+
+    # calibrate on growing prefixes (the first run pays thread start-up),
+    # then time ~target_s worth of samples
+    nb = max(64, 8 * cores)
+    dt = run(nb)
+    for goal in (0.5, 2.0):
+        nb = int(min(B, max(nb, nb * goal / max(dt, 1e-6))))
+        dt = run(nb)
+    nb = int(min(B, max(nb, nb * target_s / max(dt, 1e-6))))
+    dt = run(nb)
     return {
         "value": nb * S / dt,
         "unit": UNIT,
         "cores": cores,
         "kind": "port",
         "sample": f"first {nb} of {B} samples x S={S} beam pixels "
-                  f"({nb * S:.3e} bp.s, {dt:.1f} s wall, OpenMP over samples)",
+                  f"({nb * S:.3e} bp.s, {dt:.1f} s wall, OpenMP over samples, {cores} threads)",
     }
 
 

@@ -61,9 +61,9 @@ int todg_pack_map(const float *mp_stacked, int C, int64_t npix, int c_q, int c_u
 /* Bilinear beam-convolution TOD with spin-2 Q/U correction.
  * Replaces _gather_accum_fused_jit (tod_bilinear.py:205-222, same argument
  * order; mp_stacked becomes map_packed + C).  variant selects the kernel:
- * 0 = footprint kernel (per-sample LDS weight grid; the default), 8 = the
- * footprint kernel compiled for 4 waves/SIMD, 1/2/4 = direct kernel with
- * that many samples per wavefront.  Results agree to
+ * 0 = footprint kernel (per-sample LDS weight grid, 4 waves/SIMD; the
+ * default), 8 = the footprint kernel compiled for 2 waves/SIMD, 1/2/4 =
+ * direct kernel with that many samples per wavefront.  Results agree to
  * rounding; neighbour indices are identical. */
 int todg_gather_accum_fused(const float *vec_orig, const float *axes, const float *cos_a,
                             const float *sin_a, const float *ax_pts, const float *cos_p,

@@ -18,6 +18,9 @@ namespace todg {
 constexpr double kTwoPi = 2.0 * 3.141592653589793;  // numba_healpy.py:42
 constexpr double kPi = 3.141592653589793;
 constexpr double kTwoThirds = 2.0 / 3.0;            // numba_healpy.py:44
+// Reference longitudes lam_j = j * 2 pi / kLamN of the footprint kernel's
+// longitude frame (their cosines and sines live in a device table).
+constexpr int kLamN = 512;
 
 // One row per ring index ir in [0, 4*nside]; rows 0 and 4*nside are padding
 // so that ir indexes directly.  80 bytes.

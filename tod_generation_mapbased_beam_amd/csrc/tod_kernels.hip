@@ -148,6 +148,33 @@ int get_ring_table(int64_t nside, const RingEntry** out) {
     return TODG_OK;
 }
 
+std::mutex g_trig_mu;
+std::map<int, double*> g_trig;
+
+// cos and sin of lam_j = j * (2 pi / kLamN), glibc, per device.
+int get_lam_table(const double** out) {
+    int dev = 0;
+    TODG_HIP_CHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_trig_mu);
+    auto it = g_trig.find(dev);
+    if (it != g_trig.end()) {
+        *out = it->second;
+        return TODG_OK;
+    }
+    std::vector<double> h(2 * kLamN);
+    for (int j = 0; j < kLamN; ++j) {
+        const double lam = (double)j * (kTwoPi / kLamN);
+        h[2 * j] = cos(lam);
+        h[2 * j + 1] = sin(lam);
+    }
+    double* d = nullptr;
+    TODG_HIP_CHECK(hipMalloc(&d, h.size() * sizeof(double)));
+    TODG_HIP_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    g_trig[dev] = d;
+    *out = d;
+    return TODG_OK;
+}
+
 int cu_count() {
     static int cached = 0;
     if (cached) return cached;
@@ -174,6 +201,7 @@ struct TodArgs {
     const float* sin_p;      // (B,)
     const float* map;        // (npix, CS) pixel-major, slots permuted
     const RingEntry* rings;  // (4*nside+1,)
+    const double* lam_trig;  // (kLamN, 2): cos, sin of lam_j
     double* tod;             // (C, B), accumulated
     int64_t B;
     int64_t S;
@@ -372,138 +400,286 @@ struct Acc4 {
     double v[kMaxSlots];
 };
 
-// Direct contribution of one map pixel with weight w (spin-2 on slots 0/1
-// when QU).  Kept out of line: the footprint kernel only takes it for the
-// rare neighbour that falls outside its LDS grid.
-template <bool QU, int CS>
-__device__ __forceinline__ Acc4 direct_one(const float* __restrict__ map, int64_t npix, int nslots,
-                                        int64_t p, double z_n, double sth_n, double phi_n,
-                                        double w, double z_pts, double sth_pts,
-                                        double phi_pts) {
-    Acc4 r = {{0.0, 0.0, 0.0, 0.0}};
-    const float4 m = load_px<CS>(map, p, npix);
-    if constexpr (QU) {
-        double c2d, s2d;
-        spin2(z_n, sth_n, phi_n, z_pts, sth_pts, phi_pts, c2d, s2d);
-        const double qv = (double)m.x, uv = (double)m.y;
-        r.v[0] = w * fma(qv, c2d, uv * s2d);
-        r.v[1] = w * fma(uv, c2d, -(qv * s2d));
-#pragma unroll
-        for (int k = 2; k < CS; ++k)
-            if (k < nslots) r.v[k] = w * (double)slot(m, k);
-    } else {
-#pragma unroll
-        for (int k = 0; k < CS; ++k)
-            if (k < nslots) r.v[k] = w * (double)slot(m, k);
-    }
-    return r;
-}
-
 // ========================================================= footprint kernel
 
 constexpr int kFpThreads = 512;
 constexpr int kFpWaves = kFpThreads / 64;
 
-// Certified fast index path (fp_phase1).  A ring or slot coordinate computed
-// there differs from the reference's binary64 value by far less than 1e-10
-// (see fp_phase1); its integer part is taken only when the coordinate is at
-// least kCertSlot (kCertPolar for the polar-cap ring formula, whose slope
-// grows near the pole) away from an integer, otherwise the beam pixel is
+// Certified fast index path (fp_fast_bilinear).  A ring or slot coordinate
+// computed there differs from the reference's binary64 value by far less
+// than 1e-10; its integer part is taken only when the coordinate is at least
+// kCertSlot (kCertPolar for the polar-cap ring formula, whose slope grows
+// near the pole) away from an integer, otherwise the beam pixel is
 // recomputed by the exact path.
 constexpr double kCertSlot = 1e-9;
 constexpr double kCertPolar = 1e-6;
-// Largest |tan| of the longitude offset from the footprint centre that the
-// fast path's atan series takes (truncation t^13/13 < 1e-21 at 0.03).
-constexpr double kMaxTan = 0.03;
-
-// One footprint ring in LDS.  The fast loop reads the first 48 bytes.
-struct alignas(16) FpRing {
-    double base;      // slot coordinate of the footprint centre, window-relative
-    double inv_dphi;  // 1 / dphi
-    double z, sth, inv_dth;
-    int width, off;   // window width in slots, first weight-grid cell
-    double theta, phi0, dphi;
-    int64_t first;
-    int n, i0;        // ring pixels, first slot of the window
-};
+// Largest |tan| of the longitude offset from the reference longitude that
+// the fast path's atan series takes (truncation t^15/15 < 1e-19 at 0.06).
+constexpr double kMaxTan = 0.06;
 
 // Per-sample values shared by the workgroup (written by wave 0).
 struct alignas(16) FpFrame {
-    double m[9];   // m[3*i + j]: row i, column j of Rz(-phi_f) * R2 * R1
-    double q[12];  // Sample rotation fields: kx ky kz ca sa oma bx by bz cp sp omp
-    double z_pts, sth_pts, phi_pts;
+    double m[9];      // m[3*i + j]: row i, column j of Rz(-lam) * R2 * R1
+    double q[12];     // exact-path rotation: kx ky kz ca sa oma bx by bz cp sp omp
+    double lam;       // reference longitude
+    double z_pts, sth_pts;  // boresight z and sin(theta)  (tod_bilinear.py:306-310)
+    double sha, cha;  // sin, cos of half the boresight longitude minus lam
     int ir_lo, nr, total, pad;
 };
 
-struct FpConf {
-    int cap;        // weight-grid entries
-    int max_rings;  // FpRing entries per buffer
+// Map and geometry operands of the exact path (passed by value to the
+// out-of-line functions).
+struct FpGlobals {
+    const float* map;
+    const RingEntry* rings;
+    int64_t npix;
+    int nside;
+    int nslots;
 };
 
-// Local index of (ring entry e, slot pos in [0, n]) in the weight grid, or -1.
-__device__ __forceinline__ int fp_local(const FpRing& e, int pos) {
-    int d = pos - e.i0;
-    if (d < 0) d += e.n;
-    if (d >= e.n) d -= e.n;
-    return d < e.width ? e.off + d : -1;
+__device__ __forceinline__ FpGlobals fp_globals(const TodArgs& a) {
+    return FpGlobals{a.map, a.rings, a.npix, a.nside, a.nslots};
 }
 
-// Bilinear neighbours through the global ring table (pole caps, rings
-// outside the layout, uncertified coordinates): LDS cell when laid out,
-// direct contribution otherwise.
-template <bool QU, int CS>
-__device__ __forceinline__ Acc4 fp_slow_bilinear(const TodArgs* __restrict__ ap, double z,
-                                                 double phi_w, double bv, const FpRing* rl,
-                                                 int ir_lo, int nr, double* W, double z_pts,
-                                                 double sth_pts, double phi_pts) {
-    const TodArgs& a = *ap;
-    Nb o;
-    ring_interp(a.nside, a.rings, z, phi_w, o);
-    Acc4 r = {{0.0, 0.0, 0.0, 0.0}};
-    for (int k = 0; k < 4; ++k) {
-        const int lr = o.ring[k] - ir_lo;
-        const int li = (lr >= 0 && lr < nr) ? fp_local(rl[lr], o.pos[k]) : -1;
-        if (li >= 0) {
-            atomicAdd(&W[li], o.w[k] * bv);
-        } else {
-            const RingEntry& e = a.rings[o.ring[k]];
-            const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + o.pos[k], e.z,
-                                              e.sth, e.phi0 + (double)o.pos[k] * e.dphi,
-                                              o.w[k] * bv, z_pts, sth_pts, phi_pts);
-            for (int j = 0; j < kMaxSlots; ++j) r.v[j] += d.v[j];
+// Spin-2 inputs of one sample (uniform).
+struct FpSpin {
+    double z_pts, sth_pts, sha, cha, lam, bx, by;
+};
+
+// LDS layout of the footprint kernels (dynamic shared memory, compile-time
+// offsets, so every access is one address register plus an immediate).
+// Ring descriptors and frames are double-buffered so that wave 0 can lay
+// out sample b+1 while the other waves still read sample b's.  Ring r of
+// the footprint is global ring ir_lo + r; its window covers slots
+// i0 .. i0 + width - 1 (mod n) and grid cells off .. off + width - 1.
+extern __shared__ __align__(16) unsigned char fp_smem[];
+
+constexpr int kFpCap = 4608;       // weight-grid cells
+constexpr int kFpMaxRings = 128;   // ring descriptors per buffer
+constexpr int kRingBuf = kFpMaxRings * (7 * 8 + 5 * 4);
+constexpr int kOffRings = 0;
+constexpr int kOffFrame = kOffRings + 2 * kRingBuf;
+constexpr int kOffTrig = kOffFrame + 2 * (int)sizeof(FpFrame);
+constexpr int kOffRed = kOffTrig + 2 * kLamN * 8;
+constexpr int kOffScratch = kOffRed + kFpWaves * kMaxSlots * 8;
+constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
+constexpr int kOffW = kOffJob + 2 * 64;
+constexpr int kFpSmemBytes = kOffW + kFpCap * 8;
+
+template <typename T>
+__device__ __forceinline__ T& lds_at(unsigned off) {
+    return *reinterpret_cast<T*>(fp_smem + off);
+}
+
+// One buffer of footprint ring descriptors, structure of arrays.
+struct FpRings {
+    unsigned o;  // byte offset of the buffer
+    __device__ double& base(int r) const { return lds_at<double>(o + 8 * r); }  // slot coordinate of lam, window-relative
+    __device__ double& inv_dphi(int r) const { return lds_at<double>(o + 8 * (kFpMaxRings + r)); }
+    __device__ double& z(int r) const { return lds_at<double>(o + 8 * (2 * kFpMaxRings + r)); }
+    __device__ double& sth(int r) const { return lds_at<double>(o + 8 * (3 * kFpMaxRings + r)); }
+    __device__ double& inv_dth(int r) const { return lds_at<double>(o + 8 * (4 * kFpMaxRings + r)); }
+    __device__ double& phi0(int r) const { return lds_at<double>(o + 8 * (5 * kFpMaxRings + r)); }
+    __device__ double& dphi(int r) const { return lds_at<double>(o + 8 * (6 * kFpMaxRings + r)); }
+    __device__ int& width(int r) const { return lds_at<int>(o + 56 * kFpMaxRings + 4 * r); }
+    __device__ int& off(int r) const { return lds_at<int>(o + 60 * kFpMaxRings + 4 * r); }
+    __device__ int& i0(int r) const { return lds_at<int>(o + 64 * kFpMaxRings + 4 * r); }
+    __device__ int& n(int r) const { return lds_at<int>(o + 68 * kFpMaxRings + 4 * r); }
+    __device__ int& first(int r) const { return lds_at<int>(o + 72 * kFpMaxRings + 4 * r); }  // npix < 2^31
+};
+
+__device__ __forceinline__ FpRings fp_rings(int buf) { return FpRings{(unsigned)(kOffRings + buf * kRingBuf)}; }
+__device__ __forceinline__ FpFrame* fp_frame(int buf) {
+    return &lds_at<FpFrame>(kOffFrame + buf * (int)sizeof(FpFrame));
+}
+__device__ __forceinline__ double* fp_trig() { return &lds_at<double>(kOffTrig); }
+__device__ __forceinline__ double* fp_red() { return &lds_at<double>(kOffRed); }
+__device__ __forceinline__ double* fp_scratch() { return &lds_at<double>(kOffScratch); }
+__device__ __forceinline__ double* fp_W() { return &lds_at<double>(kOffW); }
+
+// The per-sample operands a stage needs after the sample's layout: the
+// beam, the tod destination and the sample index (one slot per buffer).
+struct alignas(16) FpJob {
+    const float* vec_orig;
+    const float* beam_vals;
+    double* tod;
+    int64_t S, B, b;
+};
+static_assert(sizeof(FpJob) <= 64, "job slot");
+__device__ __forceinline__ FpJob* fp_job(int buf) { return &lds_at<FpJob>(kOffJob + 64 * buf); }
+
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    return reinterpret_cast<T*>(__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, reinterpret_cast<uint64_t>(p)))));
+}
+
+// Local index of (footprint ring r, slot pos in [0, n]) in the weight grid, or -1.
+__device__ __forceinline__ int fp_local(const FpRings& R, int r, int pos) {
+    const int n = R.n(r);
+    int d = pos - R.i0(r);
+    if (d < 0) d += n;
+    if (d >= n) d -= n;
+    return d < R.width(r) ? R.off(r) + d : -1;
+}
+
+// sin and cos of x, |x| < 2 pi: quadrant reduction by a two-part pi/2 and
+// Taylor series on |r| <= pi/4 (truncation < 1e-19).
+__device__ __forceinline__ void sincos_red(double x, double& s, double& c) {
+    const double k = rint(x * (2.0 / kPi));
+    double r = fma(-k, 1.5707963267948966, x);
+    r = fma(-k, 6.123233995736766e-17, r);
+    const double r2 = r * r;
+    double ps = fma(r2, -1.0 / 355687428096000.0, 1.0 / 1307674368000.0);
+    ps = fma(r2, ps, -1.0 / 6227020800.0);
+    ps = fma(r2, ps, 1.0 / 39916800.0);
+    ps = fma(r2, ps, -1.0 / 362880.0);
+    ps = fma(r2, ps, 1.0 / 5040.0);
+    ps = fma(r2, ps, -1.0 / 120.0);
+    ps = fma(r2, ps, 1.0 / 6.0);
+    const double sr = fma(-(r * r2), ps, r);
+    double pc = fma(r2, 1.0 / 6402373705728000.0, -1.0 / 20922789888000.0);
+    pc = fma(r2, pc, 1.0 / 87178291200.0);
+    pc = fma(r2, pc, -1.0 / 479001600.0);
+    pc = fma(r2, pc, 1.0 / 3628800.0);
+    pc = fma(r2, pc, -1.0 / 40320.0);
+    pc = fma(r2, pc, 1.0 / 720.0);
+    pc = fma(r2, pc, -1.0 / 24.0);
+    pc = fma(r2, pc, 0.5);
+    const double cr = fma(-r2, pc, 1.0);
+    const int q = ((int)k) & 3;
+    s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+    c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+
+// Exact boresight longitude (tod_bilinear.py:306-310), for the rare exact
+// coincidence test of the spin-2 guard.
+__device__ __noinline__ double boresight_phi(double bx, double by) {
+    double ph = atan2(by, bx);
+    if (ph < 0.0) ph += kTwoPi;
+    return ph;
+}
+
+// cos 2delta, sin 2delta of one pixel centre (z_n, sth_n, phi_n) against the
+// boresight (tod_bilinear.py:84-145).  The longitude gap enters through half
+// angles: phi_pts - phi_n = (phi_pts - lam) - (phi_n - lam), with the
+// boresight half angle precomputed per sample, so no atan2 is needed.
+__device__ __forceinline__ void fp_spin2(const FpSpin& sp, double z_n, double sth_n, double phi_n,
+                                         double& c2d, double& s2d) {
+    if (z_n == sp.z_pts && sth_n == sp.sth_pts) {
+        if (boresight_phi(sp.bx, sp.by) == phi_n) {  // tod_bilinear.py:108-109
+            c2d = 1.0;
+            s2d = 0.0;
+            return;
         }
+    }
+    const double hb = (phi_n - sp.lam) * 0.5;
+    double sb, cb;
+    if (fabs(hb) <= 0.3)
+        sincos_small(hb, sb, cb);
+    else
+        sincos_red(hb, sb, cb);
+    const double sh = fma(sp.sha, cb, -(sp.cha * sb));
+    const double ch = fma(sp.cha, cb, sp.sha * sb);
+    spin2_from_half(spin2_ring(z_n, sth_n, sp.z_pts, sp.sth_pts), sh, ch, c2d, s2d);
+}
+
+// Direct contribution of one map pixel with weight w (spin-2 on slots 0/1
+// when QU), for neighbours outside the laid-out footprint.
+template <bool QU, int CS>
+__device__ __forceinline__ Acc4 fp_direct(const FpGlobals& a, const FpSpin& sp, int ring, int pos,
+                                          double w) {
+    Acc4 r = {{0.0, 0.0, 0.0, 0.0}};
+    const RingEntry& e = a.rings[ring];
+    const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
+    if constexpr (QU) {
+        double c2d, s2d;
+        fp_spin2(sp, e.z, e.sth, e.phi0 + (double)pos * e.dphi, c2d, s2d);
+        const double qv = (double)m.x, uv = (double)m.y;
+        r.v[0] = w * fma(qv, c2d, uv * s2d);
+        r.v[1] = w * fma(uv, c2d, -(qv * s2d));
+#pragma unroll
+        for (int k = 2; k < CS; ++k)
+            if (k < a.nslots) r.v[k] = w * (double)slot(m, k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < CS; ++k)
+            if (k < a.nslots) r.v[k] = w * (double)slot(m, k);
     }
     return r;
 }
 
-// LDS carve-up of the footprint kernels.  Ring descriptors and frames are
-// double-buffered so that wave 0 can lay out sample b+1 while the other
-// waves still read sample b's.
-struct FpLds {
-    FpRing* rings[2];  // fc.max_rings descriptors each
-    FpFrame* frame[2];
-    double* W;         // fc.cap weight-grid cells
-    double* red;       // kFpWaves * kMaxSlots reduction slots
-    double* scratch;   // kFpWaves doubles for fp_beam_frame
-};
+__device__ __forceinline__ FpSpin fp_spin_of(const FpFrame* fr) {
+    FpSpin sp;
+    sp.z_pts = uniform(fr->z_pts);
+    sp.sth_pts = uniform(fr->sth_pts);
+    sp.sha = uniform(fr->sha);
+    sp.cha = uniform(fr->cha);
+    sp.lam = uniform(fr->lam);
+    sp.bx = uniform(fr->q[6]);
+    sp.by = uniform(fr->q[7]);
+    return sp;
+}
 
-__device__ __forceinline__ FpLds fp_lds(unsigned char* smem, const FpConf& fc) {
-    FpLds L;
-    unsigned char* p = smem;
-    L.rings[0] = reinterpret_cast<FpRing*>(p);
-    p += (size_t)fc.max_rings * sizeof(FpRing);
-    L.rings[1] = reinterpret_cast<FpRing*>(p);
-    p += (size_t)fc.max_rings * sizeof(FpRing);
-    L.frame[0] = reinterpret_cast<FpFrame*>(p);
-    p += sizeof(FpFrame);
-    L.frame[1] = reinterpret_cast<FpFrame*>(p);
-    p += sizeof(FpFrame);
-    L.W = reinterpret_cast<double*>(p);
-    p += (size_t)fc.cap * sizeof(double);
-    L.red = reinterpret_cast<double*>(p);
-    p += (size_t)kFpWaves * kMaxSlots * sizeof(double);
-    L.scratch = reinterpret_cast<double*>(p);
-    return L;
+__device__ __forceinline__ void frame_sample(const FpFrame* fr, Sample& q) {
+    q.kx = fr->q[0];
+    q.ky = fr->q[1];
+    q.kz = fr->q[2];
+    q.ca = fr->q[3];
+    q.sa = fr->q[4];
+    q.oma = fr->q[5];
+    q.bx = fr->q[6];
+    q.by = fr->q[7];
+    q.bz = fr->q[8];
+    q.cp = fr->q[9];
+    q.sp = fr->q[10];
+    q.omp = fr->q[11];
+    make_uniform(q);
+}
+
+// Exact path for one beam pixel u (beam value bv): the reference rotation,
+// atan2 and neighbour lookup; neighbours inside the layout go to their LDS
+// cell, the others are accumulated directly.  Out of line: it runs only for
+// the rare beam pixels the fast path does not certify.
+template <int MODE, bool QU, int CS>
+__device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, float u1, float u2,
+                                            float bvf) {
+    const FpRings R = fp_rings(buf);
+    const FpFrame* fr = fp_frame(buf);
+    double* W = fp_W();
+    Sample q;
+    frame_sample(fr, q);
+    const FpSpin sp = fp_spin_of(fr);
+    const int ir_lo = fr->ir_lo, nr = fr->nr;
+    const double bv = (double)bvf;
+    double vx, vy, vz;
+    rotate(q, (double)u0, (double)u1, (double)u2, vx, vy, vz);
+    const double phi_w = wrap_phi(vy, vx);
+    Acc4 acc = {{0.0, 0.0, 0.0, 0.0}};
+    if constexpr (MODE == 0) {
+        Nb o;
+        ring_interp(a.nside, a.rings, fmax(-1.0, fmin(1.0, vz)), phi_w, o);
+        for (int k = 0; k < 4; ++k) {
+            const int lr = o.ring[k] - ir_lo;
+            const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
+            if (li >= 0) {
+                atomicAdd(&W[li], o.w[k] * bv);
+            } else {
+                const Acc4 d = fp_direct<QU, CS>(a, sp, o.ring[k], o.pos[k], o.w[k] * bv);
+                for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
+            }
+        }
+    } else {
+        const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
+        int ring, pos;
+        nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, ring, pos);
+        const int lr = ring - ir_lo;
+        const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
+        if (li >= 0)
+            atomicAdd(&W[li], bv);
+        else
+            acc = fp_direct<QU, CS>(a, sp, ring, pos, bv);
+    }
+    return acc;
 }
 
 // Beam centre c (unit vector in the beam frame) and the footprint disc
@@ -516,15 +692,32 @@ struct BeamFrame {
 // b0 and the beam radius about it.  For a beam prepared around its centre
 // every sample's rotation maps c onto its boresight; any c yields a correct
 // footprint disc around R_b c (rotations keep angles), a good c a small one.
-__device__ __forceinline__ BeamFrame fp_beam_frame(const TodArgs& a, int64_t b0, const FpLds& L) {
+struct BeamSrc {
+    const float *vec_orig, *axes, *cos_a, *sin_a, *ax_pts, *cos_p, *sin_p;
+    int64_t S;
+    int nside;
+};
+
+__device__ __forceinline__ BeamSrc beam_src(const TodArgs& a) {
+    return BeamSrc{a.vec_orig, a.axes, a.cos_a, a.sin_a, a.ax_pts, a.cos_p, a.sin_p, a.S, a.nside};
+}
+
+__device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* red) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
-    double* red = L.scratch;
     double cx, cy, cz, cosr;
     {
         Sample q;
-        load_sample(a, b0, q);
+        q.kx = (double)a.axes[3 * b0];
+        q.ky = (double)a.axes[3 * b0 + 1];
+        q.kz = (double)a.axes[3 * b0 + 2];
+        q.ca = (double)a.cos_a[b0];
+        q.sa = (double)a.sin_a[b0];
+        q.oma = 1.0 - q.ca;
+        q.bx = (double)a.ax_pts[3 * b0];
+        q.by = (double)a.ax_pts[3 * b0 + 1];
+        q.bz = (double)a.ax_pts[3 * b0 + 2];
         const double px = q.bx, py = q.by, pz = q.bz;
         const double dkv = q.kx * px + q.ky * py + q.kz * pz;
         cx = px * q.ca - (q.ky * pz - q.kz * py) * q.sa + q.kx * dkv * q.oma;
@@ -564,42 +757,59 @@ __device__ __forceinline__ BeamFrame fp_beam_frame(const TodArgs& a, int64_t b0,
 
 // Wave 0 only: sample b's frame and footprint layout -- every ring and ring
 // slot that the rotated beam's bilinear neighbours can touch, as per-ring
-// slot windows packed into the weight grid.
+// slot windows packed into the weight grid.  The windows only steer work
+// (a neighbour outside them takes the exact path), so they are estimated in
+// binary32 with a margin of three slots per side; the reference longitude
+// lam and everything the index path derives from it are binary64.
 __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const BeamFrame& bf,
-                                          const FpConf& fc, FpFrame* fr, FpRing* rings) {
+                                          const double* trig, FpFrame* fr,
+                                          FpRings R) {
     const int lane = threadIdx.x & 63;
     const int nside = a.nside;
     Sample q;
-    load_sample(a, b, q);
+    q.kx = (double)a.axes[3 * b];
+    q.ky = (double)a.axes[3 * b + 1];
+    q.kz = (double)a.axes[3 * b + 2];
+    q.ca = (double)a.cos_a[b];
+    q.sa = (double)a.sin_a[b];
+    q.oma = 1.0 - q.ca;
+    q.bx = (double)a.ax_pts[3 * b];
+    q.by = (double)a.ax_pts[3 * b + 1];
+    q.bz = (double)a.ax_pts[3 * b + 2];
+    q.cp = (double)a.cos_p[b];
+    q.sp = (double)a.sin_p[b];
+    q.omp = 1.0 - q.cp;
     make_uniform(q);
     double fx, fy, fz;  // footprint centre = this sample's image of c
     rotate(q, bf.cx, bf.cy, bf.cz, fx, fy, fz);
-    const double fzc = fmax(-1.0, fmin(1.0, fz));
-    const double fth = acos(fzc);
-    const double fst = sqrt(fmax(0.0, (1.0 - fzc) * (1.0 + fzc)));
-    double fph = atan2(fy, fx);
-    if (fph < 0.0) fph += kTwoPi;
-    const double rho = sqrt(fx * fx + fy * fy);
-    const double cf = rho > 0.0 ? fx / rho : 1.0;
-    const double sf = rho > 0.0 ? fy / rho : 0.0;
+    const float fzf = fmaxf(-1.f, fminf(1.f, (float)fz));
+    const float fthf = acosf(fzf);
+    const float fstf = sqrtf(fmaxf(0.f, (1.f - fzf) * (1.f + fzf)));
+    float fphf = atan2f((float)fy, (float)fx);
+    if (fphf < 0.f) fphf += 6.2831855f;
+    int jl = (int)rintf(fphf * (float)(kLamN / kTwoPi));
+    jl = jl >= kLamN ? jl - kLamN : jl;
+    const double lam = (double)jl * (kTwoPi / kLamN);
+    const double cl = trig[2 * jl], sl = trig[2 * jl + 1];
 
-    const double th_lo = fth - bf.R, th_hi = fth + bf.R;
-    int ir_lo = th_lo <= 0.0 ? 1 : ring_above(nside, cos(th_lo));
-    int ir_hi = th_hi >= kPi ? 4 * nside - 1 : ring_above(nside, cos(th_hi)) + 1;
+    const float Rf = (float)bf.R, cosRf = (float)bf.cosR;
+    const float th_lo = fthf - Rf, th_hi = fthf + Rf;
+    int ir_lo = th_lo <= 0.f ? 1 : ring_above(nside, (double)cosf(th_lo));
+    int ir_hi = th_hi >= 3.1415925f ? 4 * nside - 1 : ring_above(nside, (double)cosf(th_hi)) + 1;
     ir_lo = ir_lo < 1 ? 1 : ir_lo;
     ir_hi = ir_hi > 4 * nside - 1 ? 4 * nside - 1 : ir_hi;
     int nr = ir_hi - ir_lo + 1;
-    if (nr > fc.max_rings || nr < 0) nr = 0;  // no layout: everything goes direct
+    if (nr > kFpMaxRings || nr < 0) nr = 0;  // no layout: everything goes direct
 
     // Composite rotation: column j of R2 R1 is the reference rotation of the
-    // unit vector e_j; the rows are then turned by -phi_f about z, so that
-    // the rotated direction's longitude is phi_f + atan2(row1.v, row0.v).
+    // unit vector e_j; its rows are then turned by -lam about z, so that the
+    // rotated direction's longitude is lam + atan2(row1.v, row0.v).
     if (lane < 3) {
         double mx, my, mz;
         rotate(q, lane == 0 ? 1.0 : 0.0, lane == 1 ? 1.0 : 0.0, lane == 2 ? 1.0 : 0.0, mx, my,
                mz);
-        fr->m[lane] = fma(cf, mx, sf * my);
-        fr->m[3 + lane] = fma(cf, my, -(sf * mx));
+        fr->m[lane] = fma(cl, mx, sl * my);
+        fr->m[3 + lane] = fma(cl, my, -(sl * mx));
         fr->m[6 + lane] = mz;
     }
 
@@ -607,33 +817,25 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
     for (int r0 = 0; r0 < nr; r0 += 64) {
         const int r = r0 + lane;
         int width = 0, i0 = 0;
-        double base = 0.0;
-        FpRing e;
         if (r < nr) {
             const RingEntry& g = a.rings[ir_lo + r];
-            e.inv_dphi = g.inv_dphi;
-            e.z = g.z;
-            e.sth = g.sth;
-            e.inv_dth = g.inv_dth;
-            e.theta = g.theta;
-            e.phi0 = g.phi0;
-            e.dphi = g.dphi;
-            e.first = g.first;
-            e.n = g.n;
-            const double tc = (fph - g.phi0) * g.inv_dphi;  // centre's slot coordinate
-            const double den = fst * g.sth;
-            const double cd = den > 1e-12 ? (bf.cosR - fzc * g.z) / den : -2.0;
+            const float nf = (float)g.n;
+            const double tc = (lam - g.phi0) * g.inv_dphi;  // lam's slot coordinate
+            const float den = fstf * (float)g.sth;
+            const float cd = den > 1e-6f ? (cosRf - fzf * (float)g.z) / den : -2.f;
+            double base = 0.0;
             bool full = false;
-            if (cd <= -1.0) {
+            if (cd <= -1.f) {
                 full = true;
-            } else if (cd < 1.0) {
-                const double dp = acos(cd);
-                const double wf = 2.0 * dp / g.dphi + 6.0;
-                if (wf >= (double)g.n) {
+            } else if (cd < 1.f) {
+                const float dp = acosf(cd);
+                const float spr = nf * (float)(1.0 / kTwoPi);  // slots per radian
+                const float wf = 2.f * dp * spr + 6.f;
+                if (wf >= nf) {
                     full = true;
                 } else {
                     width = (int)wf;
-                    const int f = (int)floor((fph - dp - g.phi0) / g.dphi) - 2;
+                    const int f = (int)floorf((fphf - dp - (float)g.phi0) * spr) - 2;
                     base = tc - (double)f;
                     const int fm = f % g.n;
                     i0 = fm < 0 ? fm + g.n : fm;
@@ -641,9 +843,17 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
             }
             if (full) {
                 width = g.n;
-                i0 = 0;
                 base = tc >= (double)g.n ? tc - (double)g.n : (tc < 0.0 ? tc + (double)g.n : tc);
             }
+            R.base(r) = base;
+            R.inv_dphi(r) = g.inv_dphi;
+            R.z(r) = g.z;
+            R.sth(r) = g.sth;
+            R.inv_dth(r) = g.inv_dth;
+            R.phi0(r) = g.phi0;
+            R.dphi(r) = g.dphi;
+            R.n(r) = g.n;
+            R.first(r) = (int)g.first;
         }
         int incl = width;  // inclusive scan of widths across the wave
 #pragma unroll
@@ -652,14 +862,12 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
             if (lane >= d) incl += t;
         }
         const int off = carry + incl - width;
-        const bool fits = off + width <= fc.cap;
+        const bool fits = off + width <= kFpCap;
         int end = (r < nr && fits) ? off + width : 0;
         if (r < nr) {
-            e.base = base;
-            e.i0 = i0;
-            e.width = fits ? width : 0;  // beyond capacity: accumulated directly
-            e.off = fits ? off : 0;
-            rings[r] = e;
+            R.i0(r) = i0;
+            R.width(r) = fits ? width : 0;  // beyond capacity: accumulated directly
+            R.off(r) = fits ? off : 0;
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
@@ -667,6 +875,25 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         carry += __shfl(incl, 63, 64);
     }
     if (lane == 0) {
+        // boresight half angle relative to lam, from the rotated boresight
+        // (x, y) = Rz(-lam) (bx, by): cos a = x/rho, sin a = y/rho
+        const double xb = fma(cl, q.bx, sl * q.by);
+        const double yb = fma(cl, q.by, -(sl * q.bx));
+        const double rho = sqrt(fma(xb, xb, yb * yb));
+        double sha, cha;
+        if (rho > 0.0) {
+            const double ca = xb / rho, sa = yb / rho;
+            if (ca >= 0.0) {
+                cha = sqrt(0.5 * (1.0 + ca));
+                sha = sa / (2.0 * cha);
+            } else {
+                sha = sqrt(0.5 * (1.0 - ca));
+                if (sa < 0.0) sha = -sha;
+                cha = sa / (2.0 * sha);
+            }
+        } else {  // boresight on the axis: the reference's atan2(0, 0) = 0
+            sincos_red(-0.5 * lam, sha, cha);
+        }
         fr->q[0] = q.kx;
         fr->q[1] = q.ky;
         fr->q[2] = q.kz;
@@ -679,32 +906,15 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         fr->q[9] = q.cp;
         fr->q[10] = q.sp;
         fr->q[11] = q.omp;
-        fr->z_pts = q.z_pts;
-        fr->sth_pts = q.sth_pts;
-        fr->phi_pts = q.phi_pts;
+        fr->lam = lam;
+        fr->z_pts = fmax(-1.0, fmin(1.0, q.bz));
+        fr->sth_pts = sqrt(fmax(0.0, 1.0 - q.bz * q.bz));
+        fr->sha = sha;
+        fr->cha = cha;
         fr->ir_lo = ir_lo;
         fr->nr = nr;
         fr->total = laid;  // grid cells [0, laid) belong to rings
     }
-}
-
-__device__ __forceinline__ void frame_sample(const FpFrame* fr, Sample& q) {
-    q.kx = fr->q[0];
-    q.ky = fr->q[1];
-    q.kz = fr->q[2];
-    q.ca = fr->q[3];
-    q.sa = fr->q[4];
-    q.oma = fr->q[5];
-    q.bx = fr->q[6];
-    q.by = fr->q[7];
-    q.bz = fr->q[8];
-    q.cp = fr->q[9];
-    q.sp = fr->q[10];
-    q.omp = fr->q[11];
-    q.z_pts = fr->z_pts;
-    q.sth_pts = fr->sth_pts;
-    q.phi_pts = fr->phi_pts;
-    make_uniform(q);
 }
 
 // sqrt(x) for x > 0 from the hardware reciprocal square root and one
@@ -722,7 +932,7 @@ __device__ __forceinline__ double sqrt_fast(double x) {
 //
 // Index certification.  The rotated direction comes from the composite
 // matrix instead of two sequential Rodrigues steps, and its longitude as
-// phi_f + atan(t) instead of atan2; both differ from the reference's binary64
+// lam + atan(t) instead of atan2; both differ from the reference's binary64
 // values by a few ulp (|dz| < 1e-15, |dphi| < 1e-15 for sin(theta) > 0.05).
 // A ring coordinate nside*(2 - 1.5 z) then differs by < 1e-11, a polar one
 // nside*sqrt(3(1-|z|)) by < 1e-7, and a slot coordinate (phi - phi0)/dphi by
@@ -733,9 +943,10 @@ __device__ __forceinline__ double sqrt_fast(double x) {
 // or slots outside the laid-out windows, large longitude offsets -- goes to
 // the exact path.  Weights only need tolerance accuracy.
 __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, int nside,
-                                                 int ir_lo, int nr, const FpRing* rings,
-                                                 double* W, double u0, double u1, double u2,
-                                                 double bv) {
+                                                 int ir_lo, int nr, const FpRings& R, double* W,
+                                                 double u0, double u1, double u2, double bv) {
+    // (conditions are combined with & rather than && so that they compile to
+    // masks instead of branches; every LDS index is clamped into range first)
     const double xp = fma(m[0], u0, fma(m[1], u1, m[2] * u2));
     const double yp = fma(m[3], u0, fma(m[4], u1, m[5] * u2));
     const double zr = fma(m[6], u0, fma(m[7], u1, m[8] * u2));
@@ -748,48 +959,47 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
         tr = fma(z, -1.5 * (double)nside, 2.0 * (double)nside);
         ira = (int)tr;
         const double fr = tr - (double)ira;
-        ok = fr > kCertSlot && fr < 1.0 - kCertSlot;
+        ok = (fr > kCertSlot) & (fr < 1.0 - kCertSlot);
     } else {
         tr = (double)nside * sqrt_fast(3.0 * (1.0 - az));
         const int it = (int)tr;
         ira = z < 0.0 ? 4 * nside - it - 1 : it;
         const double fr = tr - (double)it;
-        ok = fr > kCertPolar && fr < 1.0 - kCertPolar;
+        ok = (fr > kCertPolar) & (fr < 1.0 - kCertPolar);
     }
-    ok = ok && fabs(az - kTwoThirds) > 1e-12;
-    const int la = ira - ir_lo;
-    ok = ok && ira >= 1 && ira <= 4 * nside - 2 && la >= 0 && la + 1 < nr;
-    ok = ok && xp > 0.0 && fabs(yp) < kMaxTan * xp;
-    if (!ok) return false;
-    const FpRing& A = rings[la];
-    const FpRing& Bn = rings[la + 1];
-    // longitude offset from the footprint centre: atan(t), |t| < 0.03
+    ok &= fabs(az - kTwoThirds) > 1e-12;
+    int la = ira - ir_lo;
+    ok &= (ira >= 1) & (ira <= 4 * nside - 2) & (la >= 0) & (la + 1 < nr);
+    ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
+    la = ok ? la : 0;
+    // longitude offset from lam: atan(t), |t| < 0.06
     const double t = yp * rcp_nr(xp);
     const double t2 = t * t;
-    double p = fma(t2, -1.0 / 11.0, 1.0 / 9.0);
+    double p = fma(t2, 1.0 / 13.0, -1.0 / 11.0);
+    p = fma(t2, p, 1.0 / 9.0);
     p = fma(t2, p, -1.0 / 7.0);
     p = fma(t2, p, 1.0 / 5.0);
     p = fma(t2, p, -1.0 / 3.0);
     const double dl = fma(t * t2, p, t);
-    const double ta = fma(dl, A.inv_dphi, A.base);
-    const double tb = fma(dl, Bn.inv_dphi, Bn.base);
+    const double ta = fma(dl, R.inv_dphi(la), R.base(la));
+    const double tb = fma(dl, R.inv_dphi(la + 1), R.base(la + 1));
     const int ia = (int)ta, ib = (int)tb;
     const double fa = ta - (double)ia, fb = tb - (double)ib;
-    ok = ta >= 0.0 && tb >= 0.0 && ia + 1 < A.width && ib + 1 < Bn.width;
-    ok = ok && fa > kCertSlot && fa < 1.0 - kCertSlot && fb > kCertSlot && fb < 1.0 - kCertSlot;
+    ok &= (ta >= 0.0) & (tb >= 0.0) & (ia + 1 < R.width(la)) & (ib + 1 < R.width(la + 1));
+    ok &= (fa > kCertSlot) & (fa < 1.0 - kCertSlot) & (fb > kCertSlot) & (fb < 1.0 - kCertSlot);
     // theta - theta_above = asin(sin(theta) z_a - z sin(theta_a))
     const double sq = sqrt_fast((1.0 - z) * (1.0 + z));
-    const double x = fma(sq, A.z, -(z * A.sth));
-    ok = ok && fabs(x) < 1e-2;
+    const double x = fma(sq, R.z(la), -(z * R.sth(la)));
+    ok &= fabs(x) < 1e-2;
     if (!ok) return false;
     const double x2 = x * x;
     double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
     pp = fma(x2, pp, 3.0 / 40.0);
     pp = fma(x2, pp, 1.0 / 6.0);
-    const double wb = fma(x * x2, pp, x) * A.inv_dth * bv;
+    const double wb = fma(x * x2, pp, x) * R.inv_dth(la) * bv;
     const double wa = bv - wb;
-    double* ca = W + A.off + ia;
-    double* cb = W + Bn.off + ib;
+    double* ca = W + R.off(la) + ia;
+    double* cb = W + R.off(la + 1) + ib;
     atomicAdd(ca, wa - wa * fa);
     atomicAdd(ca + 1, wa * fa);
     atomicAdd(cb, wb - wb * fb);
@@ -797,93 +1007,58 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     return true;
 }
 
-// All threads: phase 1 of one sample -- sweep the beam and scatter
+// Waves 1..7: phase 1 of one sample -- sweep the beam and scatter
 // beam_val * weight into the LDS weight grid; returns the outside-grid sums.
+// (Wave 0 meanwhile lays out the next sample.)
+constexpr int kP1Threads = kFpThreads - 64;
+
 template <int MODE, bool QU, int CS>
-__device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpFrame* fr,
-                                          const FpRing* rings, double* W) {
-    const int tid = threadIdx.x;
+__device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, const FpFrame* fr,
+                                          const FpRings& R, int buf, double* W) {
+    const int tid = threadIdx.x - 64;
     const int nside = a.nside;
+    const float* __restrict__ vec = uniform_ptr(J->vec_orig);
+    const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
+    const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     double m[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
-    Sample qn;  // nearest mode rotates with the exact path
-    if constexpr (MODE != 0) frame_sample(fr, qn);
     Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};  // outside-grid sums
-    for (int64_t s0 = 0; s0 < a.S; s0 += (int64_t)kFpThreads * 64) {
+    for (int64_t s0 = 0; s0 < S; s0 += (int64_t)kP1Threads * 64) {
         uint64_t slow = 0;
         float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
         int64_t s = s0 + tid;
-        if (s < a.S) {
-            nv0 = a.vec_orig[3 * s];
-            nv1 = a.vec_orig[3 * s + 1];
-            nv2 = a.vec_orig[3 * s + 2];
-            nbv = a.beam_vals[s];
+        if (s < S) {
+            nv0 = vec[3 * s];
+            nv1 = vec[3 * s + 1];
+            nv2 = vec[3 * s + 2];
+            nbv = bvs[s];
         }
-        for (int it = 0; it < 64 && s < a.S; ++it, s += kFpThreads) {
+        for (int it = 0; it < 64 && s < S; ++it, s += kP1Threads) {
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
             const double bv = (double)nbv;
-            const int64_t sn = s + kFpThreads;
-            if (it + 1 < 64 && sn < a.S) {  // fetch the next beam pixel early
-                nv0 = a.vec_orig[3 * sn];
-                nv1 = a.vec_orig[3 * sn + 1];
-                nv2 = a.vec_orig[3 * sn + 2];
-                nbv = a.beam_vals[sn];
+            const int64_t sn = s + kP1Threads;
+            if (it + 1 < 64 && sn < S) {  // fetch the next beam pixel early
+                nv0 = vec[3 * sn];
+                nv1 = vec[3 * sn + 1];
+                nv2 = vec[3 * sn + 2];
+                nbv = bvs[sn];
             }
             bool ok = false;
-            if constexpr (MODE == 0) {
-                ok = fp_fast_bilinear(m, nside, ir_lo, nr, rings, W, v0, v1, v2, bv);
-            } else {
-                double vx, vy, vz;
-                rotate(qn, v0, v1, v2, vx, vy, vz);
-                const double phi_w = wrap_phi(vy, vx);
-                const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-                int ring, pos;
-                nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
-                const int lr = ring - ir_lo;
-                const int li = (lr >= 0 && lr < nr) ? fp_local(rings[lr], pos) : -1;
-                if (li >= 0) {
-                    atomicAdd(&W[li], bv);
-                    ok = true;
-                }
-            }
+            if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
             if (!ok) slow |= 1ull << it;
         }
-        // exact path for the flagged beam pixels
-        if (slow) {
-            Sample q;
-            frame_sample(fr, q);
-            while (slow) {
-                const int it = __builtin_ctzll(slow);
-                slow &= slow - 1;
-                const int64_t ss = s0 + (int64_t)it * kFpThreads + tid;
-                const double v0 = (double)a.vec_orig[3 * ss];
-                const double v1 = (double)a.vec_orig[3 * ss + 1];
-                const double v2 = (double)a.vec_orig[3 * ss + 2];
-                const double bv = (double)a.beam_vals[ss];
-                double vx, vy, vz;
-                rotate(q, v0, v1, v2, vx, vy, vz);
-                const double phi_w = wrap_phi(vy, vx);
-                if constexpr (MODE == 0) {
-                    const Acc4 d = fp_slow_bilinear<QU, CS>(&a, fmax(-1.0, fmin(1.0, vz)), phi_w,
-                                                            bv, rings, ir_lo, nr, W, q.z_pts,
-                                                            q.sth_pts, q.phi_pts);
+        // exact path for the beam pixels not certified above
+        while (slow) {
+            const int it = __builtin_ctzll(slow);
+            slow &= slow - 1;
+            const int64_t ss = s0 + (int64_t)it * kP1Threads + tid;
+            const Acc4 d = fp_exact_pixel<MODE, QU, CS>(fp_globals(a), buf, vec[3 * ss],
+                                                        vec[3 * ss + 1], vec[3 * ss + 2], bvs[ss]);
 #pragma unroll
-                    for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
-                } else {
-                    const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
-                    int ring, pos;
-                    nearest_pix(nside, a.rings, vz, sin_th, phi_w, ring, pos);
-                    const RingEntry& e = a.rings[ring];
-                    const Acc4 d = direct_one<QU, CS>(a.map, a.npix, a.nslots, e.first + pos, e.z,
-                                                      e.sth, e.phi0 + (double)pos * e.dphi, bv,
-                                                      q.z_pts, q.sth_pts, q.phi_pts);
-#pragma unroll
-                    for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
-                }
-            }
+            for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
         }
     }
     return accd;
@@ -893,37 +1068,25 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpFrame* fr,
 // gather per map pixel with nonzero weight; the cell is cleared for the next
 // sample.  Returns the per-thread sums (outside-grid sums included).
 template <bool QU, int CS>
-__device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr,
-                                          const FpRing* rings, double* W, Acc4 acc) {
+__device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
+                                          double* W, Acc4 acc) {
     const int tid = threadIdx.x;
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     const int total = __builtin_amdgcn_readfirstlane(fr->total);
-    const double z_pts = uniform(fr->z_pts), sth_pts = uniform(fr->sth_pts);
-    const double phi_pts = uniform(fr->phi_pts);
+    const FpSpin sp = fp_spin_of(fr);
     int r = 0;
     for (int j = tid; j < total; j += kFpThreads) {
-        while (r + 1 < nr && j >= rings[r].off + rings[r].width) ++r;
+        while (r + 1 < nr && j >= R.off(r) + R.width(r)) ++r;
         const double wsum = W[j];
         if (wsum == 0.0) continue;
         W[j] = 0.0;
-        const FpRing& e = rings[r];
-        int pos = e.i0 + (j - e.off);
-        if (pos >= e.n) pos -= e.n;
-        const float4 mv = load_px<CS>(a.map, e.first + pos, a.npix);
+        int pos = R.i0(r) + (j - R.off(r));
+        const int n = R.n(r);
+        if (pos >= n) pos -= n;
+        const float4 mv = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
         if constexpr (QU) {
-            const double phi_n = e.phi0 + (double)pos * e.dphi;
             double c2d, s2d;
-            if (phi_pts == phi_n && sth_pts == e.sth && z_pts == e.z) {
-                c2d = 1.0;
-                s2d = 0.0;
-            } else {
-                double h = (phi_pts - phi_n) * 0.5;
-                if (h > 0.5 * kPi) h -= kPi;
-                if (h < -0.5 * kPi) h += kPi;
-                double sh, ch;
-                sincos_half(h, sh, ch);
-                spin2_from_half(spin2_ring(e.z, e.sth, z_pts, sth_pts), sh, ch, c2d, s2d);
-            }
+            fp_spin2(sp, R.z(r), R.sth(r), R.phi0(r) + (double)pos * R.dphi(r), c2d, s2d);
             const double qv = (double)mv.x, uv = (double)mv.y;
             acc.v[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc.v[0]);
             acc.v[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
@@ -939,71 +1102,80 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr,
     return acc;
 }
 
-// A sample whose block sums sit in red[] until thread 0 writes them.
-struct FpPending {
-    double* tod;
-    int64_t B, b;
-};
-
-// Thread 0: add the pending sample's block sums to its tod entries, slots in
-// order (two slots may share a row when c_q == c_u).  Callers guarantee that
-// every wave's red[] entries are complete (a barrier since they were made).
-__device__ __forceinline__ void fp_flush(const TodArgs& a, const FpLds& L, FpPending& pend) {
-    if (threadIdx.x == 0 && pend.tod) {
+// Thread 0: add a finished sample's block sums (red[], complete since the
+// last barrier) to its tod entries, slots in order (two slots may share a
+// row when c_q == c_u).  Each tod entry belongs to one sample and each sample
+// to one workgroup, so the atomic adds have no contention and leave the
+// result deterministic; they need no reply, so nothing waits on them.
+__device__ __forceinline__ void fp_flush(const TodArgs& a, const FpJob* J) {
+    if (threadIdx.x == 0) {
+        double* tod = J->tod;
+        const int64_t B = J->B, b = J->b;
         for (int k = 0; k < kMaxSlots && k < a.nslots; ++k) {
             double sum = 0.0;
-            for (int w = 0; w < kFpWaves; ++w) sum += L.red[w * kMaxSlots + k];
-            pend.tod[(int64_t)a.row[k] * pend.B + pend.b] += sum;
+            for (int w = 0; w < kFpWaves; ++w) sum += fp_red()[w * kMaxSlots + k];
+            atomicAdd(&tod[(int64_t)a.row[k] * B + b], sum);
         }
     }
-    pend.tod = nullptr;
 }
 
-// All threads of the block: one sample through layout, both phases and the
-// block reduction; two barriers per sample.  The tod entries of sample b are
-// written after the next sample's first barrier (or by fp_drain).
+// Zero the weight grid and copy the reference-longitude table into LDS.
+__device__ __forceinline__ void fp_init(const double* lam_trig) {
+    for (int j = threadIdx.x; j < kFpCap; j += kFpThreads) fp_W()[j] = 0.0;
+    for (int j = threadIdx.x; j < 2 * kLamN; j += kFpThreads) fp_trig()[j] = lam_trig[j];
+    __syncthreads();
+}
+
+// Thread 0: record sample b of item a in job slot buf.
+__device__ __forceinline__ void fp_set_job(const TodArgs& a, int64_t b, int buf) {
+    if (threadIdx.x == 0) {
+        FpJob* J = fp_job(buf);
+        J->vec_orig = a.vec_orig;
+        J->beam_vals = a.beam_vals;
+        J->tod = a.tod;
+        J->S = a.S;
+        J->B = a.B;
+        J->b = b;
+    }
+}
+
+// The software pipeline of the footprint kernels, one step: wave 0 lays out
+// the next sample (if any, item `an`, index bn, beam frame bfn) into the
+// other buffer while waves 1..7 run phase 1 of the current sample; after a
+// barrier all waves run phase 2 and the block reduction; after a second
+// barrier thread 0 adds the sums to tod.  Two barriers per sample, and the
+// layout's latency hides behind phase 1.
 template <int MODE, bool QU, int CS>
-__device__ __forceinline__ void fp_sample(const TodArgs& a, int64_t b, const BeamFrame& bf,
-                                          const FpConf& fc, const FpLds& L, int& buf,
-                                          FpPending& pend) {
+__device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64_t bn,
+                                        const BeamFrame& bfn, int buf) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    FpFrame* fr = L.frame[buf];
-    FpRing* rings = L.rings[buf];
-    if (wave == 0) fp_layout(a, b, bf, fc, fr, rings);
+    const FpRings R = fp_rings(buf);
+    const FpFrame* fr = fp_frame(buf);
+    Acc4 acc = {{0.0, 0.0, 0.0, 0.0}};
+    if (wave == 0) {
+        if (have_next) {
+            fp_set_job(an, bn, buf ^ 1);
+            fp_layout(an, bn, bfn, fp_trig(), fp_frame(buf ^ 1), fp_rings(buf ^ 1));
+        }
+    } else {
+        acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
+    }
     __syncthreads();
-    fp_flush(a, L, pend);
-    Acc4 acc = fp_phase1<MODE, QU, CS>(a, fr, rings, L.W);
-    __syncthreads();
-    acc = fp_phase2<QU, CS>(a, fr, rings, L.W, acc);
+    acc = fp_phase2<QU, CS>(an, fr, R, fp_W(), acc);
 #pragma unroll
     for (int k = 0; k < kMaxSlots; ++k) {
         if (k < CS) {
             const double v = wave_sum(acc.v[k]);
-            if (lane == 0) L.red[wave * kMaxSlots + k] = v;
+            if (lane == 0) fp_red()[wave * kMaxSlots + k] = v;
         }
     }
-    pend.tod = a.tod;
-    pend.B = a.B;
-    pend.b = b;
-    buf ^= 1;
-}
-
-__device__ __forceinline__ void fp_drain(const TodArgs& a, const FpLds& L, FpPending& pend) {
     __syncthreads();
-    fp_flush(a, L, pend);
-    __syncthreads();
-}
-
-__device__ __forceinline__ void fp_clear_grid(const FpConf& fc, const FpLds& L) {
-    for (int j = threadIdx.x; j < fc.cap; j += kFpThreads) L.W[j] = 0.0;
-    __syncthreads();
+    fp_flush(an, fp_job(buf));
 }
 
 template <int MODE, bool QU, int CS, int MINW>
-__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, const FpConf fc) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const FpLds L = fp_lds(smem, fc);
+__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     // XCD-aware sample order: workgroups are dispatched round-robin over the
     // 8 XCDs, so block bid runs on XCD bid % 8.  Each XCD gets a contiguous
     // eighth of the samples, so concurrently processed samples on one XCD
@@ -1016,13 +1188,23 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a, co
     const int64_t bbeg = (int64_t)xcd * chunk;
     const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
     if (bbeg + kx >= bend) return;  // uniform over the block
-    fp_clear_grid(fc, L);
-    const BeamFrame bf = fp_beam_frame(a, bbeg + kx, L);
+    fp_init(a.lam_trig);
+    const BeamFrame bf = fp_beam_frame(beam_src(a), bbeg + kx, fp_scratch());
+    int64_t b = bbeg + kx;
+    if ((threadIdx.x >> 6) == 0) {
+        fp_set_job(a, b, 0);
+        fp_layout(a, b, bf, fp_trig(), fp_frame(0), fp_rings(0));
+    }
+    __syncthreads();
     int buf = 0;
-    FpPending pend = {nullptr, 0, 0};
-    for (int64_t b = bbeg + kx; b < bend; b += per)
-        fp_sample<MODE, QU, CS>(a, b, bf, fc, L, buf, pend);
-    fp_drain(a, L, pend);
+    for (;;) {
+        const int64_t bn = b + per;
+        const bool have_next = bn < bend;
+        fp_step<MODE, QU, CS>(a, have_next, bn, bf, buf);
+        if (!have_next) break;
+        b = bn;
+        buf ^= 1;
+    }
 }
 
 // ======================================================= batched kernel
@@ -1034,6 +1216,7 @@ struct BatchArgs {
     int64_t total;                // sum of items[i].B
     const float* map;
     const RingEntry* rings;
+    const double* lam_trig;
     int64_t npix;
     int nside;
     int nslots;
@@ -1072,74 +1255,101 @@ __device__ __forceinline__ bool finite_sample(const TodArgs& a, int64_t b) {
 // chunk lengths balance themselves.  Every workgroup exits when all eight
 // queues are drained.  A sample with a non-finite pointing operand is
 // skipped and flags its item's status word.
-template <int MODE, bool QU, int CS, int MINW>
-__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchArgs ba,
-                                                                   const FpConf fc) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const FpLds L = fp_lds(smem, fc);
+struct BatchCursor {
+    int att;          // queues tried so far
+    int64_t g, g1;    // current claim [g, g1)
+    int cur;          // item of `a`
+    bool have_frame;
+    BeamFrame bf;
+};
+
+// All threads of the block (uniform): advance to the next sample to process;
+// loads its item into `a`, its beam frame into c.bf.  Returns false when
+// every queue is drained.
+__device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, TodArgs& a,
+                                           int64_t& b) {
     const int tid = threadIdx.x;
     const int xcd = blockIdx.x & 7;
     const int64_t qlen = (ba.total + 7) / 8;
-    int cur = -1;
+    int* claim = reinterpret_cast<int*>(fp_scratch() + kFpWaves);
+    for (;;) {
+        while (c.g >= c.g1) {
+            if (c.att >= 8) return false;
+            const int qi = (xcd + c.att) & 7;
+            const int64_t qbeg = (int64_t)qi * qlen;
+            const int64_t qend = (qbeg + qlen < ba.total) ? qbeg + qlen : ba.total;
+            if (tid == 0) *claim = (int)atomicAdd(&ba.queue[qi], (unsigned int)ba.chunk);
+            __syncthreads();
+            const int64_t g0 = qbeg + (int64_t)(unsigned int)*claim;
+            __syncthreads();
+            if (g0 >= qend) {
+                ++c.att;
+                continue;
+            }
+            c.g = g0;
+            c.g1 = (g0 + ba.chunk < qend) ? g0 + ba.chunk : qend;
+        }
+        const int64_t g = c.g++;
+        const int it = find_item(ba.items, ba.n_items, g);
+        const todg_work_item& w = ba.items[it];
+        if (it != c.cur) {
+            a.vec_orig = w.vec_orig;
+            a.beam_vals = w.beam_vals;
+            a.axes = w.axes;
+            a.cos_a = w.cos_a;
+            a.sin_a = w.sin_a;
+            a.ax_pts = w.ax_pts;
+            a.cos_p = w.cos_p;
+            a.sin_p = w.sin_p;
+            a.tod = w.tod;
+            a.B = w.B;
+            a.S = w.S;
+            c.cur = it;
+            c.have_frame = false;
+        }
+        b = g - w.first;
+        if (a.S <= 0) continue;
+        if (!finite_sample(a, b)) {
+            if (tid == 0 && ba.status) atomicOr(&ba.status[it], 1);
+            continue;
+        }
+        if (!c.have_frame) {  // beam frame from the item's first finite sample seen here
+            c.bf = fp_beam_frame(beam_src(a), b, fp_scratch());
+            c.have_frame = true;
+        }
+        return true;
+    }
+}
+
+template <int MODE, bool QU, int CS, int MINW>
+__global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchArgs ba,
+                                                                   int unused) {
     TodArgs a;
     a.map = ba.map;
     a.rings = ba.rings;
+    a.lam_trig = ba.lam_trig;
     a.npix = ba.npix;
     a.nside = ba.nside;
     a.nslots = ba.nslots;
 #pragma unroll
     for (int k = 0; k < kMaxSlots; ++k) a.row[k] = ba.row[k];
-    BeamFrame bf = {0.0, 0.0, 1.0, 0.0, 1.0};
-    bool have_frame = false;
-    int buf = 0;
-    FpPending pend = {nullptr, 0, 0};
-    fp_clear_grid(fc, L);
-    int* claim = reinterpret_cast<int*>(L.scratch + kFpWaves);
-    for (int att = 0; att < 8;) {
-        const int qi = (xcd + att) & 7;
-        const int64_t qbeg = (int64_t)qi * qlen;
-        const int64_t qend = (qbeg + qlen < ba.total) ? qbeg + qlen : ba.total;
-        if (tid == 0) *claim = (int)atomicAdd(&ba.queue[qi], (unsigned int)ba.chunk);
-        __syncthreads();
-        const int64_t g0 = qbeg + (int64_t)(unsigned int)*claim;
-        __syncthreads();
-        if (g0 >= qend) {
-            ++att;
-            continue;
-        }
-        const int64_t g1 = (g0 + ba.chunk < qend) ? g0 + ba.chunk : qend;
-        for (int64_t g = g0; g < g1; ++g) {
-            const int it = find_item(ba.items, ba.n_items, g);
-            const todg_work_item& w = ba.items[it];
-            if (it != cur) {
-                a.vec_orig = w.vec_orig;
-                a.beam_vals = w.beam_vals;
-                a.axes = w.axes;
-                a.cos_a = w.cos_a;
-                a.sin_a = w.sin_a;
-                a.ax_pts = w.ax_pts;
-                a.cos_p = w.cos_p;
-                a.sin_p = w.sin_p;
-                a.tod = w.tod;
-                a.B = w.B;
-                a.S = w.S;
-                cur = it;
-                have_frame = false;
-            }
-            const int64_t b = g - w.first;
-            if (a.S <= 0) continue;
-            if (!finite_sample(a, b)) {
-                if (tid == 0 && ba.status) atomicOr(&ba.status[it], 1);
-                continue;
-            }
-            if (!have_frame) {  // beam frame from the item's first finite sample seen here
-                bf = fp_beam_frame(a, b, L);
-                have_frame = true;
-            }
-            fp_sample<MODE, QU, CS>(a, b, bf, fc, L, buf, pend);
-        }
+    fp_init(a.lam_trig);
+    BatchCursor c = {0, 0, 0, -1, false, {0.0, 0.0, 1.0, 0.0, 1.0}};
+    int64_t b = 0;
+    if (!batch_next(ba, c, a, b)) return;
+    if ((threadIdx.x >> 6) == 0) {
+        fp_set_job(a, b, 0);
+        fp_layout(a, b, c.bf, fp_trig(), fp_frame(0), fp_rings(0));
     }
-    fp_drain(a, L, pend);
+    __syncthreads();
+    int buf = 0;
+    for (;;) {
+        int64_t bn = 0;
+        const bool have_next = batch_next(ba, c, a, bn);
+        fp_step<MODE, QU, CS>(a, have_next, bn, c.bf, buf);
+        if (!have_next) break;
+        buf ^= 1;
+    }
 }
 
 __global__ void k_zero_u32(unsigned int* p, int n) {
@@ -1242,18 +1452,6 @@ int slot_layout(int C, int c_q, int c_u, bool* qu, int perm[kMaxSlots], int row[
     return TODG_OK;
 }
 
-// Footprint-kernel LDS configuration: two buffers of ring descriptors and
-// frames, the weight grid, reduction and scratch slots (~62 KB per
-// workgroup: two workgroups of 8 waves per CU).
-constexpr int kFpMaxRings = 128;
-constexpr int kFpCap = 4608;
-
-size_t fp_smem_bytes(const FpConf& fc) {
-    return 2 * (size_t)fc.max_rings * sizeof(FpRing) + 2 * sizeof(FpFrame) +
-           (size_t)fc.cap * sizeof(double) + (size_t)kFpWaves * kMaxSlots * sizeof(double) +
-           (size_t)(kFpWaves + 2) * sizeof(double);
-}
-
 template <int MODE, bool QU, int CS>
 hipError_t launch_direct_cs(const TodArgs& a, int T, hipStream_t st) {
     const int64_t ntiles = (a.B + T - 1) / T;
@@ -1272,8 +1470,7 @@ hipError_t launch_direct_cs(const TodArgs& a, int T, hipStream_t st) {
 // one workgroup per CU) or 4 (128 VGPRs, two workgroups per CU).
 template <int MODE, bool QU, int CS, int MINW>
 hipError_t launch_fp_cs(const TodArgs& a, hipStream_t st) {
-    const FpConf fc{kFpCap, kFpMaxRings};
-    const size_t smem = fp_smem_bytes(fc);
+    const size_t smem = kFpSmemBytes;
     static std::once_flag once;
     std::call_once(once, [&] {
         (void)hipFuncSetAttribute((const void*)k_tod_fp<MODE, QU, CS, MINW>,
@@ -1286,14 +1483,13 @@ hipError_t launch_fp_cs(const TodArgs& a, hipStream_t st) {
     if (blocks > need) blocks = need;
     blocks = ((blocks + 7) / 8) * 8;
     hipLaunchKernelGGL((k_tod_fp<MODE, QU, CS, MINW>), dim3((unsigned)blocks), dim3(kFpThreads),
-                       smem, st, a, fc);
+                       smem, st, a);
     return hipGetLastError();
 }
 
 template <int MODE, bool QU, int CS, int MINW>
 hipError_t launch_batch_cs(const BatchArgs& ba, hipStream_t st) {
-    const FpConf fc{kFpCap, kFpMaxRings};
-    const size_t smem = fp_smem_bytes(fc);
+    const size_t smem = kFpSmemBytes;
     static std::once_flag once;
     std::call_once(once, [&] {
         (void)hipFuncSetAttribute((const void*)k_tod_fp_batch<MODE, QU, CS, MINW>,
@@ -1303,14 +1499,14 @@ hipError_t launch_batch_cs(const BatchArgs& ba, hipStream_t st) {
     int64_t blocks = (int64_t)cu_count() * (MINW >= 4 ? 2 : 1) * 2;
     blocks = ((blocks + 7) / 8) * 8;
     hipLaunchKernelGGL((k_tod_fp_batch<MODE, QU, CS, MINW>), dim3((unsigned)blocks),
-                       dim3(kFpThreads), smem, st, ba, fc);
+                       dim3(kFpThreads), smem, st, ba, 0);
     return hipGetLastError();
 }
 
 template <int MODE, bool QU, int CS>
 hipError_t launch_batch_v(const BatchArgs& ba, int variant, hipStream_t st) {
-    if (variant == 8) return launch_batch_cs<MODE, QU, CS, 4>(ba, st);
-    return launch_batch_cs<MODE, QU, CS, 2>(ba, st);
+    if (variant == 8) return launch_batch_cs<MODE, QU, CS, 2>(ba, st);
+    return launch_batch_cs<MODE, QU, CS, 4>(ba, st);
 }
 
 template <int MODE>
@@ -1328,8 +1524,8 @@ template <int MODE, bool QU, int CS>
 hipError_t launch_cs(const TodArgs& a, int variant, hipStream_t st) {
     if (variant == 1 || variant == 2 || variant == 4)
         return launch_direct_cs<MODE, QU, CS>(a, variant, st);
-    if (variant == 8) return launch_fp_cs<MODE, QU, CS, 4>(a, st);
-    return launch_fp_cs<MODE, QU, CS, 2>(a, st);
+    if (variant == 8) return launch_fp_cs<MODE, QU, CS, 2>(a, st);
+    return launch_fp_cs<MODE, QU, CS, 4>(a, st);
 }
 
 template <int MODE>
@@ -1361,6 +1557,10 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
     const RingEntry* rings = nullptr;
     rc = get_ring_table(nside, &rings);
     if (rc) return rc;
+    const double* trig = nullptr;
+    rc = get_lam_table(&trig);
+    if (rc) return rc;
+    a->lam_trig = trig;
     a->vec_orig = vec_orig;
     a->beam_vals = beam_vals;
     a->axes = axes;
@@ -1483,6 +1683,10 @@ int todg_gather_accum_batch(int mode, const todg_work_item* items, int n_items,
     const RingEntry* rings = nullptr;
     rc = get_ring_table(nside, &rings);
     if (rc) return rc;
+    const double* trig = nullptr;
+    rc = get_lam_table(&trig);
+    if (rc) return rc;
+    ba.lam_trig = trig;
     ba.items = items;
     ba.n_items = n_items;
     ba.total = total_samples;
