@@ -79,6 +79,25 @@ int todg_gather_accum_nearest(const float *vec_orig, const float *axes, const fl
                               int C, const float *beam_vals, int64_t B, int64_t S,
                               double *tod, int c_q, int c_u, int variant, void *stream);
 
+/* Beam-convolution TOD with the bicubic judge: the same rotation and spin-2
+ * as todg_gather_accum_fused, each beam pixel interpolated with Keys
+ * (Catmull-Rom, a = -0.5) weights on the gnomonic tangent plane over the
+ * reference's 7-ring x 5-pixel stencil (numba_healpy.py:892-977; the
+ * weighting follows README.md:120 -- the reference's own bicubic module,
+ * tod_bicubic.py on its bicubic branch, is not in its snapshot).  Same
+ * arguments as todg_gather_accum_fused; variant 0 or 8 (footprint kernel). */
+int todg_gather_accum_bicubic(const float *vec_orig, const float *axes, const float *cos_a,
+                              const float *sin_a, const float *ax_pts, const float *cos_p,
+                              const float *sin_p, int64_t nside, const float *map_packed,
+                              int C, const float *beam_vals, int64_t B, int64_t S,
+                              double *tod, int c_q, int c_u, int variant, void *stream);
+
+/* Point values of one map field (map: npix float32, RING order) at N
+ * directions (z, phi) (device float64 arrays): mode 0 = bilinear
+ * (numba_healpy.py:358-499), 1 = the bicubic judge above.  out: N float64. */
+int todg_interp_points(int mode, int64_t nside, const float *map, int64_t N, const double *z,
+                       const double *phi, double *out, void *stream);
+
 /* One detector (or one detector's scan chunk) of a batched launch.  All
  * pointers are device pointers with the layouts above; `first` is the
  * global index of this item's first sample, i.e. the sum of B over the
@@ -93,7 +112,7 @@ typedef struct todg_work_item {
 /* Many detectors with their own beams (uneven S) and scan chunks (uneven B)
  * in one persistent launch (the reference calls beam_tod_batch once per
  * beam entry per batch: sample_based_tod_generation_gridint.py:284-296).
- * mode 0 = bilinear, 1 = nearest; items: device array of n_items; the map,
+ * mode 0 = bilinear, 1 = nearest, 2 = bicubic; items: device array of n_items; the map,
  * C and c_q/c_u are shared.  chunk = samples per work-queue claim (1..65536);
  * at most 2^33 samples per launch.
  * queue: device scratch of >= 8 unsigned ints (zeroed by the call itself,
@@ -129,6 +148,51 @@ int todg_nearest_indices(const float *vec_orig, const float *axes, const float *
  * NULL. */
 int todg_ring_table(int64_t nside, double *z, double *theta, double *phi0, double *dphi,
                     int64_t *first, int64_t *n);
+
+
+/* ------------------------------------------------------------------------
+ * Learned interpolation corrector (no counterpart in the reference: the
+ * build's corrector of bilinear interpolation toward the bicubic judge).
+ * An MLP 18 -> 64 (GELU) -> 64 (GELU) -> 16 with FP32 master parameters in
+ * one flat array of TODG_CORR_NPARAMS floats: W1 (64 x 18), b1, W2 (64 x 64),
+ * b2, W3 (16 x 64), b3 (row-major, output x input).  Matrix products run on
+ * BF16 MFMA with FP32 accumulation.  All pointers are device pointers. */
+#define TODG_CORR_NPARAMS 6416
+
+/* Features of N query directions (z, phi: float64) on one RING-ordered map
+ * field (npix float32): X (N x 18: fa - 1/2, v - 1/2, the 4 x 4
+ * neighbourhood normalised by its mean and RMS), wbil (N x 4: bilinear
+ * weights of neighbourhood cells 5, 6, 9, 10), mu, sigma (N), pix (N x 16
+ * pixel indices) and ok (N: 0 where the neighbourhood leaves the normal
+ * ring range and the corrector does not apply). */
+int todg_corr_features(int64_t nside, const float *field, int64_t N, const double *z,
+                       const double *phi, float *X, float *wbil, float *mu, float *sigma,
+                       int *pix, int *ok, void *stream);
+
+/* Float count of the workspace for N rows of todg_corr_forward / _loss_grad. */
+size_t todg_corr_workspace_floats(int64_t N);
+
+/* O (N x 16) = the MLP's raw outputs for feature rows X. */
+int todg_corr_forward(const float *params, const float *X, int64_t N, float *ws, float *O,
+                      void *stream);
+
+/* Loss sum_i (sum_k w_ik xhat_ik - that_i)^2 over the N rows (into *loss,
+ * float64, may be NULL), its gradient with respect to the parameters
+ * divided by N (into grad, TODG_CORR_NPARAMS floats, may be NULL) and the
+ * normalised predictions (into pred, N floats, may be NULL). */
+int todg_corr_loss_grad(const float *params, const float *X, const float *wbil,
+                        const float *that, int64_t N, float *ws, float *grad, double *loss,
+                        float *pred, void *stream);
+
+/* One Adam step (bias-corrected, step >= 1) on the FP32 master parameters. */
+int todg_corr_adam(float *params, const float *grad, float *m, float *v, int step, float lr,
+                   float beta1, float beta2, float eps, void *stream);
+
+/* C (M x N) = alpha op(A) op(B) + beta C on BF16 MFMA (row-major float32
+ * operands rounded to bfloat16; ta / tb transpose A / B). */
+int todg_corr_gemm_bf16(int M, int N, int K, const float *A, int lda, int ta, const float *B,
+                        int ldb, int tb, float *C, int ldc, float alpha, float beta,
+                        void *stream);
 
 #ifdef __cplusplus
 }

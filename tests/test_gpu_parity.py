@@ -186,5 +186,11 @@ def test_beam_tod_batch_dropin():
         ref = _cpu_tod(mode, c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2)
         got = np.stack([out[k] for k in (0, 1, 2)]).astype(np.float64)
         assert rel_err(got, ref.astype(np.float32)) <= 2e-6
+    out = beam_tod_batch(c["nside"], None, data, rot, ph, th, psis, interp_mode="bicubic")
+    params = R._rotation_params(rot, ph, th, psis)
+    ref = np.zeros((3, B))
+    O.gather_accum_bicubic(c["vec"], *params, c["nside"], c["map"], c["bv"], B, c["S"], ref, 1, 2)
+    got = np.stack([out[k] for k in (0, 1, 2)]).astype(np.float64)
+    assert rel_err(got, ref.astype(np.float32)) <= 2e-6
     with pytest.raises(ValueError):
-        beam_tod_batch(c["nside"], None, data, rot, ph, th, psis, interp_mode="bicubic")
+        beam_tod_batch(c["nside"], None, data, rot, ph, th, psis, interp_mode="gaussian")

@@ -43,6 +43,19 @@ SIGNATURES = {
         _i32,
         [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _i32, _p],
     ),
+    "todg_gather_accum_bicubic": (
+        _i32,
+        [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _i32, _p],
+    ),
+    "todg_interp_points": (_i32, [_i32, _i64, _p, _i64, _p, _p, _p, _p]),
+    "todg_corr_features": (_i32, [_i64, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "todg_corr_workspace_floats": (ctypes.c_size_t, [_i64]),
+    "todg_corr_forward": (_i32, [_p, _p, _i64, _p, _p, _p]),
+    "todg_corr_loss_grad": (_i32, [_p, _p, _p, _p, _i64, _p, _p, _p, _p, _p]),
+    "todg_corr_adam": (_i32, [_p, _p, _p, _p, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, _p]),
+    "todg_corr_gemm_bf16": (_i32, [_i32, _i32, _i32, _p, _i32, _i32, _p, _i32, _i32, _p, _i32,
+                                    ctypes.c_float, ctypes.c_float, _p]),
     "todg_bilinear_indices": (
         _i32,
         [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _i64, _i64, _i64, _p, _p, _p],

@@ -40,9 +40,11 @@ class Detector:
 class BatchPlan:
     def __init__(self, detectors, dmap: DeviceMap, mode: str = "bilinear", chunk: int = 8,
                  variant: int = 0):
-        if mode not in ("bilinear", "nearest"):
-            raise ValueError(f"interp_mode {mode!r} not available; use 'bilinear' or 'nearest'")
-        self.mode = 0 if mode == "bilinear" else 1
+        modes = {"bilinear": 0, "nearest": 1, "bicubic": 2}
+        if mode not in modes:
+            raise ValueError(f"interp_mode {mode!r} not available; use 'bilinear', 'nearest' "
+                             "or 'bicubic'")
+        self.mode = modes[mode]
         self.dmap = dmap
         self.chunk = int(chunk)
         self.variant = int(variant)

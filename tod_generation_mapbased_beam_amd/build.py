@@ -23,8 +23,8 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_NAME = "libtodgpu.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ["tod_kernels.hip"]
-HEADERS = ["healpix_ring.hpp"]
+SOURCES = ["tod_kernels.hip", "corrector.hip"]
+HEADERS = ["healpix_ring.hpp", "internal.hpp"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -1,0 +1,416 @@
+// Learned interpolation corrector: features, BF16-MFMA training kernels
+// and the C-ABI that drives them.
+//
+// The corrector is a small MLP.  For a query direction it sees the
+// sub-pixel position of the direction in its bilinear cell and the 4 x 4
+// map neighbourhood around that cell (4 rings x 4 slots), normalised by the
+// neighbourhood's mean and RMS; it emits 16 weight corrections whose mean is
+// removed, so that the corrected weights (bilinear + correction) still sum to
+// one and reproduce constant maps exactly.  It is trained to turn bilinear
+// interpolation into the bicubic judge (bicubic_visit).
+//
+//   x   = [fa - 1/2, v - 1/2, (m_k - mu) / sigma for k < 16]     (18)
+//   h1  = gelu(W1 x + b1)        (64)
+//   h2  = gelu(W2 h1 + b2)       (64)
+//   o   = W3 h2 + b3             (16)
+//   w   = w_bilinear + o - mean(o)
+//   prediction = mu + sigma * sum_k w_k (m_k - mu) / sigma
+//
+// Matrix products run on MFMA in BF16 (v_mfma_f32_16x16x16_bf16: operands
+// rounded to bfloat16, FP32 accumulation); biases, GELU, the loss and the
+// Adam update are FP32 on the FP32 master weights.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include <string>
+
+#include "healpix_ring.hpp"
+#include "internal.hpp"
+#include "../../include/todgpu.h"
+
+using namespace todg;
+
+namespace todg_corr {
+
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kIn = 18, kH = 64, kOut = 16;
+// flat parameter layout (FP32): W1 (kH x kIn), b1, W2 (kH x kH), b2, W3 (kOut x kH), b3
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kH * kIn;
+constexpr int kOffW2 = kOffB1 + kH;
+constexpr int kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH;
+constexpr int kOffB3 = kOffW3 + kOut * kH;
+constexpr int kNParams = kOffB3 + kOut;
+static_assert(kNParams == TODG_CORR_NPARAMS, "parameter count");
+
+// fp32 -> bf16, round to nearest even (finite inputs).
+__device__ __forceinline__ short bf16_bits(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+// GELU, tanh form, and its derivative.
+__device__ __forceinline__ float gelu(float z) {
+    const float c = 0.7978845608028654f;  // sqrt(2/pi)
+    const float t = tanhf(c * fmaf(0.044715f * z * z, z, z));
+    return 0.5f * z * (1.f + t);
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+    const float c = 0.7978845608028654f;
+    const float t = tanhf(c * fmaf(0.044715f * z * z, z, z));
+    const float dt = (1.f - t * t) * c * (1.f + 3.f * 0.044715f * z * z);
+    return 0.5f * (1.f + t) + 0.5f * z * dt;
+}
+
+// C (M x N, row-major, ldc) = alpha * op(A) op(B) + beta * C with
+// op(A)[i][k] = ta ? A[k*lda + i] : A[i*lda + k] (M x K) and
+// op(B)[k][j] = tb ? B[j*ldb + k] : B[k*ldb + j] (K x N).  One wave per
+// 16 x 16 tile of C, K in steps of 16 (zero padded).  MFMA 16x16x16 operand
+// layout: lane l holds A[l%16][4(l/16) .. +3], B[4(l/16) .. +3][l%16], and
+// C[4(l/16) .. +3][l%16].
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const float* __restrict__ A,
+                                                   int lda, int ta, const float* __restrict__ B,
+                                                   int ldb, int tb, float* __restrict__ C, int ldc,
+                                                   float alpha, float beta) {
+    const int lane = threadIdx.x & 63;
+    const int tiles_n = (N + 15) / 16;
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int64_t tiles = (int64_t)((M + 15) / 16) * tiles_n;
+    if (tile >= tiles) return;
+    const int ti = (int)(tile / tiles_n), tj = (int)(tile % tiles_n);
+    const int ia = ti * 16 + (lane & 15);
+    const int jb = tj * 16 + (lane & 15);
+    const int kq = (lane >> 4) * 4;
+    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        short4_t av, bv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + kq + q;
+            float x = 0.f, y = 0.f;
+            if (ia < M && k < K) x = ta ? A[(int64_t)k * lda + ia] : A[(int64_t)ia * lda + k];
+            if (jb < N && k < K) y = tb ? B[(int64_t)jb * ldb + k] : B[(int64_t)k * ldb + jb];
+            av[q] = bf16_bits(x);
+            bv[q] = bf16_bits(y);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = ti * 16 + kq + q;
+        if (i < M && jb < N) {
+            float* c = C + (int64_t)i * ldc + jb;
+            *c = beta == 0.f ? alpha * acc[q] : fmaf(alpha, acc[q], beta * *c);
+        }
+    }
+}
+
+// Z[i][f] += b[f]; A = gelu(Z) (Z kept for the backward pass).
+__global__ __launch_bounds__(256) void k_bias_gelu(int64_t rows, int F, const float* __restrict__ b,
+                                                   float* __restrict__ Z, float* __restrict__ A) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * F) return;
+    const float z = Z[i] + b[i % F];
+    Z[i] = z;
+    A[i] = gelu(z);
+}
+
+// O[i][f] += b[f].
+__global__ __launch_bounds__(256) void k_bias(int64_t rows, int F, const float* __restrict__ b,
+                                              float* __restrict__ O) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * F) return;
+    O[i] += b[i % F];
+}
+
+// dZ = dA * gelu'(Z) (in place on dA).
+__global__ __launch_bounds__(256) void k_gelu_bwd(int64_t n, const float* __restrict__ Z,
+                                                  float* __restrict__ dA) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dA[i] *= gelu_grad(Z[i]);
+}
+
+// Head: corrected weights, normalised prediction and residual per row;
+// dO = (2 / N) r (m_hat - mean(m_hat)); per-block partial sums of r^2.
+__global__ __launch_bounds__(256) void k_head(int64_t N, const float* __restrict__ X,
+                                              const float* __restrict__ wbil,
+                                              const float* __restrict__ that,
+                                              const float* __restrict__ O, float* __restrict__ dO,
+                                              float* __restrict__ pred, double* __restrict__ part) {
+    __shared__ double red[256];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double r2 = 0.0;
+    if (i < N) {
+        const float* x = X + i * kIn + 2;
+        const float* o = O + i * kOut;
+        float om = 0.f, mm = 0.f;
+#pragma unroll
+        for (int k = 0; k < kOut; ++k) {
+            om += o[k];
+            mm += x[k];
+        }
+        om *= 1.f / kOut;
+        mm *= 1.f / kOut;
+        const float* wb = wbil + i * 4;
+        float p = wb[0] * x[5] + wb[1] * x[6] + wb[2] * x[9] + wb[3] * x[10];
+#pragma unroll
+        for (int k = 0; k < kOut; ++k) p = fmaf(o[k] - om, x[k], p);
+        if (pred) pred[i] = p;
+        const float r = p - that[i];
+        r2 = (double)r * (double)r;
+        if (dO) {
+            const float g = 2.f * r / (float)N;
+#pragma unroll
+            for (int k = 0; k < kOut; ++k) dO[i * kOut + k] = g * (x[k] - mm);
+        }
+    }
+    red[threadIdx.x] = r2;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part) part[blockIdx.x] = red[0];
+}
+
+// out[f] = sum_i D[i][f] (bias gradients), one thread per column.
+__global__ __launch_bounds__(256) void k_colsum(int64_t rows, int F, const float* __restrict__ D,
+                                                float* __restrict__ out) {
+    __shared__ float red[256];
+    const int f = blockIdx.x;
+    float s = 0.f;
+    for (int64_t i = threadIdx.x; i < rows; i += blockDim.x) s += D[i * F + f];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[f] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_parts(int n, const double* __restrict__ part,
+                                                   double* __restrict__ out) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// Adam on the FP32 master weights (bias-corrected, step t >= 1).
+__global__ __launch_bounds__(256) void k_adam(int n, float* __restrict__ w, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, float lr,
+                                              float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// Features of N query directions (z, phi) on one map field: the bilinear
+// cell (numba_healpy.py:443-480: rings ira, ira+1, slots ia, ib, fractions
+// fa, fb, weight v of the lower ring) and the 4 x 4 neighbourhood rings
+// ira-1 .. ira+2 x slots j-1 .. j+2 (j = the ring's bilinear slot).
+// X (N x 18), wbil (N x 4: weights of the cells 5, 6, 9, 10), mu, sigma,
+// pix (N x 16) and ok (0 where the neighbourhood leaves the normal branch:
+// the corrector is not applied there).
+__global__ __launch_bounds__(256) void k_features(int nside, const RingEntry* __restrict__ rt,
+                                                  const float* __restrict__ field, int64_t N,
+                                                  const double* __restrict__ z,
+                                                  const double* __restrict__ phi,
+                                                  float* __restrict__ X, float* __restrict__ wbil,
+                                                  float* __restrict__ mu, float* __restrict__ sigma,
+                                                  int* __restrict__ pix, int* __restrict__ ok) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double zz = fmax(-1.0, fmin(1.0, z[i]));
+    const double pp = phi[i];
+    const int ira = ring_above(nside, zz);
+    const bool good = ira >= 2 && ira <= 4 * nside - 3;
+    int p16[16];
+    double fa = 0.0, v = 0.0, fb = 0.0;
+    if (good) {
+        Nb o;
+        ring_interp(nside, rt, zz, pp, o);
+        const double wa = o.w[0] + o.w[1];
+        v = o.w[2] + o.w[3];
+        fa = wa > 0.0 ? o.w[1] / wa : 0.0;
+        fb = v > 0.0 ? o.w[3] / v : 0.0;
+        for (int r = 0; r < 4; ++r) {
+            const RingEntry& e = rt[ira - 1 + r];
+            int j;
+            if (r == 1)
+                j = o.pos[0];
+            else if (r == 2)
+                j = o.pos[2];
+            else {
+                double fr;
+                j = ring_slot(pp, e.phi0, e.dphi, e.inv_dphi, e.n, fr);
+            }
+            for (int s = 0; s < 4; ++s) {
+                int q = j - 1 + s;
+                q = ((q % e.n) + e.n) % e.n;
+                p16[4 * r + s] = (int)e.first + q;
+            }
+        }
+    } else {
+        for (int k = 0; k < 16; ++k) p16[k] = 0;
+    }
+    float m[16];
+    double s1 = 0.0;
+    for (int k = 0; k < 16; ++k) {
+        m[k] = field[p16[k]];
+        s1 += m[k];
+    }
+    const double mean = s1 / 16.0;
+    double s2 = 0.0;
+    for (int k = 0; k < 16; ++k) s2 += ((double)m[k] - mean) * ((double)m[k] - mean);
+    const double sd = sqrt(s2 / 16.0);
+    const double inv = sd > 1e-30 ? 1.0 / sd : 0.0;
+    float* x = X + i * kIn;
+    x[0] = (float)(fa - 0.5);
+    x[1] = (float)(v - 0.5);
+    for (int k = 0; k < 16; ++k) x[2 + k] = (float)(((double)m[k] - mean) * inv);
+    wbil[i * 4 + 0] = (float)((1.0 - v) * (1.0 - fa));
+    wbil[i * 4 + 1] = (float)((1.0 - v) * fa);
+    wbil[i * 4 + 2] = (float)(v * (1.0 - fb));
+    wbil[i * 4 + 3] = (float)(v * fb);
+    mu[i] = (float)mean;
+    sigma[i] = (float)sd;
+    for (int k = 0; k < 16; ++k) pix[i * 16 + k] = p16[k];
+    ok[i] = good ? 1 : 0;
+}
+
+int fail(int code, const std::string& msg) { return todg_internal::set_error(code, msg); }
+
+inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+void gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, int ta, const float* B,
+          int ldb, int tb, float* C, int ldc, float alpha, float beta) {
+    const int64_t tiles = (int64_t)((M + 15) / 16) * ((N + 15) / 16);
+    hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, M, N, K,
+                       A, lda, ta, B, ldb, tb, C, ldc, alpha, beta);
+}
+
+// Forward pass into the workspace: Z1, A1 (N x kH), Z2, A2 (N x kH), O (N x kOut).
+void forward(hipStream_t st, const float* P, const float* X, int64_t N, float* Z1, float* A1,
+             float* Z2, float* A2, float* O) {
+    const int n = (int)N;
+    gemm(st, n, kH, kIn, X, kIn, 0, P + kOffW1, kIn, 1, Z1, kH, 1.f, 0.f);
+    hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(N * kH)), dim3(256), 0, st, N, kH, P + kOffB1, Z1, A1);
+    gemm(st, n, kH, kH, A1, kH, 0, P + kOffW2, kH, 1, Z2, kH, 1.f, 0.f);
+    hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(N * kH)), dim3(256), 0, st, N, kH, P + kOffB2, Z2, A2);
+    gemm(st, n, kOut, kH, A2, kH, 0, P + kOffW3, kH, 1, O, kOut, 1.f, 0.f);
+    hipLaunchKernelGGL(k_bias, dim3(nblk(N * kOut)), dim3(256), 0, st, N, kOut, P + kOffB3, O);
+}
+
+}  // namespace todg_corr
+
+using namespace todg_corr;
+
+extern "C" {
+
+size_t todg_corr_workspace_floats(int64_t N) { return (size_t)N * (4 * kH + 2 * kOut + 1) + 1024; }
+
+int todg_corr_features(int64_t nside, const float* field, int64_t N, const double* z,
+                       const double* phi, float* X, float* wbil, float* mu, float* sigma, int* pix,
+                       int* ok, void* stream) {
+    if (N < 0 || nside < 1 || nside > 8192 || (nside & (nside - 1)))
+        return fail(TODG_ERR_ARG, "bad nside or N");
+    if (N == 0) return TODG_OK;
+    const RingEntry* rt = nullptr;
+    int rc = todg_internal::ring_table(nside, &rt);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_features, dim3(nblk(N)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       (int)nside, rt, field, N, z, phi, X, wbil, mu, sigma, pix, ok);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_corr_forward(const float* params, const float* X, int64_t N, float* ws, float* O,
+                      void* stream) {
+    if (N <= 0 || N > (1 << 30)) return fail(TODG_ERR_ARG, "N out of range");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    float* Z1 = ws;
+    float* A1 = Z1 + N * kH;
+    float* Z2 = A1 + N * kH;
+    float* A2 = Z2 + N * kH;
+    forward(st, params, X, N, Z1, A1, Z2, A2, O);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_corr_loss_grad(const float* params, const float* X, const float* wbil, const float* that,
+                        int64_t N, float* ws, float* grad, double* loss, float* pred,
+                        void* stream) {
+    if (N <= 0 || N > (1 << 30)) return fail(TODG_ERR_ARG, "N out of range");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int n = (int)N;
+    float* Z1 = ws;
+    float* A1 = Z1 + N * kH;
+    float* Z2 = A1 + N * kH;
+    float* A2 = Z2 + N * kH;
+    float* O = A2 + N * kH;
+    float* dO = O + N * kOut;
+    double* part = reinterpret_cast<double*>(dO + N * kOut);  // 8-byte aligned: 288 N floats
+    forward(st, params, X, N, Z1, A1, Z2, A2, O);
+    const unsigned hb = nblk(N);
+    hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, N, X, wbil, that, O, grad ? dO : nullptr,
+                       pred, part);
+    if (loss) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (int)hb, part, loss);
+    if (grad) {
+        // layer 3: dW3 = dO^T A2, db3 = colsum dO, dA2 = dO W3
+        gemm(st, kOut, kH, n, dO, kOut, 1, A2, kH, 0, grad + kOffW3, kH, 1.f, 0.f);
+        hipLaunchKernelGGL(k_colsum, dim3(kOut), dim3(256), 0, st, N, kOut, dO, grad + kOffB3);
+        gemm(st, n, kH, kOut, dO, kOut, 0, params + kOffW3, kH, 0, A2, kH, 1.f, 0.f);
+        hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * kH)), dim3(256), 0, st, N * kH, Z2, A2);
+        // layer 2 (dZ2 in A2): dW2 = dZ2^T A1, db2, dA1 = dZ2 W2
+        gemm(st, kH, kH, n, A2, kH, 1, A1, kH, 0, grad + kOffW2, kH, 1.f, 0.f);
+        hipLaunchKernelGGL(k_colsum, dim3(kH), dim3(256), 0, st, N, kH, A2, grad + kOffB2);
+        gemm(st, n, kH, kH, A2, kH, 0, params + kOffW2, kH, 0, A1, kH, 1.f, 0.f);
+        hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * kH)), dim3(256), 0, st, N * kH, Z1, A1);
+        // layer 1 (dZ1 in A1): dW1 = dZ1^T X, db1
+        gemm(st, kH, kIn, n, A1, kH, 1, X, kIn, 0, grad + kOffW1, kIn, 1.f, 0.f);
+        hipLaunchKernelGGL(k_colsum, dim3(kH), dim3(256), 0, st, N, kH, A1, grad + kOffB1);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_corr_adam(float* params, const float* grad, float* m, float* v, int step, float lr,
+                   float beta1, float beta2, float eps, void* stream) {
+    if (step < 1) return fail(TODG_ERR_ARG, "step must be >= 1");
+    const float c1 = 1.f - powf(beta1, (float)step);
+    const float c2 = 1.f - powf(beta2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(kNParams)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       kNParams, params, grad, m, v, lr, beta1, beta2, eps, c1, c2);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_corr_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const float* B,
+                        int ldb, int tb, float* C, int ldc, float alpha, float beta,
+                        void* stream) {
+    if (M <= 0 || N <= 0 || K <= 0) return fail(TODG_ERR_ARG, "empty product");
+    gemm(reinterpret_cast<hipStream_t>(stream), M, N, K, A, lda, ta, B, ldb, tb, C, ldc, alpha, beta);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+}  // extern "C"

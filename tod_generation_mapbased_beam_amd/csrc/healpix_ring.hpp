@@ -117,6 +117,69 @@ __device__ __forceinline__ void sincos_small(double x, double& s, double& c) {
     c = fma(x2, pc, 1.0);
 }
 
+// sin and cos of x, |x| < 2 pi: quadrant reduction by a two-part pi/2 and
+// Taylor series on |r| <= pi/4 (truncation < 1e-19).
+__device__ __forceinline__ void sincos_red(double x, double& s, double& c) {
+    const double k = rint(x * (2.0 / kPi));
+    double r = fma(-k, 1.5707963267948966, x);
+    r = fma(-k, 6.123233995736766e-17, r);
+    const double r2 = r * r;
+    double ps = fma(r2, -1.0 / 355687428096000.0, 1.0 / 1307674368000.0);
+    ps = fma(r2, ps, -1.0 / 6227020800.0);
+    ps = fma(r2, ps, 1.0 / 39916800.0);
+    ps = fma(r2, ps, -1.0 / 362880.0);
+    ps = fma(r2, ps, 1.0 / 5040.0);
+    ps = fma(r2, ps, -1.0 / 120.0);
+    ps = fma(r2, ps, 1.0 / 6.0);
+    const double sr = fma(-(r * r2), ps, r);
+    double pc = fma(r2, 1.0 / 6402373705728000.0, -1.0 / 20922789888000.0);
+    pc = fma(r2, pc, 1.0 / 87178291200.0);
+    pc = fma(r2, pc, -1.0 / 479001600.0);
+    pc = fma(r2, pc, 1.0 / 3628800.0);
+    pc = fma(r2, pc, -1.0 / 40320.0);
+    pc = fma(r2, pc, 1.0 / 720.0);
+    pc = fma(r2, pc, -1.0 / 24.0);
+    pc = fma(r2, pc, 0.5);
+    const double cr = fma(-r2, pc, 1.0);
+    const int q = ((int)k) & 3;
+    s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
+    c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
+}
+
+// sin and cos of |x| <= 0.05 by Taylor series (truncation < 1e-20).
+__device__ __forceinline__ void sincos_tiny(double x, double& s, double& c) {
+    const double x2 = x * x;
+    double ps = fma(x2, -1.0 / 5040.0, 1.0 / 120.0);
+    ps = fma(x2, ps, -1.0 / 6.0);
+    s = fma(x * x2, ps, x);
+    double pc = fma(x2, 1.0 / 40320.0, -1.0 / 720.0);
+    pc = fma(x2, pc, 1.0 / 24.0);
+    pc = fma(x2, pc, -0.5);
+    c = fma(x2, pc, 1.0);
+}
+
+// sin and cos of 0.05 < |x| < 2 pi, out of line so that the hot callers do
+// not carry its coefficients.
+__device__ __noinline__ double2 sincos_wide(double x) {
+    double s, c;
+    if (fabs(x) <= 0.3)
+        sincos_small(x, s, c);
+    else
+        sincos_red(x, s, c);
+    return make_double2(s, c);
+}
+
+// sin and cos of |x| < 2 pi: Taylor when small, else quadrant-reduced.
+__device__ __forceinline__ void sincos_any(double x, double& s, double& c) {
+    if (fabs(x) <= 0.05) {
+        sincos_tiny(x, s, c);
+    } else {
+        const double2 r = sincos_wide(x);
+        s = r.x;
+        c = r.y;
+    }
+}
+
 // sin and cos of the half angle x; only products of pairs are used
 // downstream, so (s, c) and (-s, -c) are interchangeable and x may be
 // reduced by pi.
@@ -333,6 +396,73 @@ __device__ __forceinline__ void nearest_pix(int nside, const RingEntry* __restri
             }
         }
     }
+}
+
+
+// ------------------------------------------------------------ bicubic judge
+
+// Keys cubic convolution kernel, a = -0.5 (Catmull-Rom), support |t| < 2.
+__device__ __forceinline__ double keys_kernel(double t) {
+    t = fabs(t);
+    if (t < 1.0) return (1.5 * t - 2.5) * t * t + 1.0;
+    if (t < 2.0) return ((-0.5 * t + 2.5) * t - 4.0) * t + 2.0;
+    return 0.0;
+}
+
+// The bicubic judge's candidates and unnormalised weights for a query
+// (z, phi): the reference's 7-ring x 5-pixel stencil (numba_healpy.py:938-976)
+// around the query, each candidate weighted by K(x) K(y), where (x, y) are its
+// gnomonic coordinates on the tangent plane at the query, x in units of the
+// candidate's own ring step dphi*sin(theta), y in units of the ring spacing
+// theta(ira+1) - theta(ira) at the query (README.md:120: Keys/Catmull-Rom
+// weights via gnomonic projection; the reference's module itself is not in
+// its snapshot).  Calls f(ring, slot, w) for every candidate with w != 0 and
+// returns the sum of the weights.  sin/cos of the longitude gap come from
+// sincos_any, so no absolute-angle trigonometry is needed.
+template <class F>
+__device__ __forceinline__ double bicubic_visit(int nside, const RingEntry* __restrict__ rt,
+                                                double z, double phi, F&& f) {
+    const double st = sqrt(fmax(0.0, (1.0 - z) * (1.0 + z)));
+    int ira = ring_above(nside, z);
+    ira = ira < 1 ? 1 : (ira > 4 * nside - 2 ? 4 * nside - 2 : ira);
+    const double dth = rt[ira + 1].theta - rt[ira].theta;
+    int irc = ring_above(nside, z);
+    irc = irc < 1 ? 1 : (irc > 4 * nside - 1 ? 4 * nside - 1 : irc);
+    const int r_lo = irc - 3 > 1 ? irc - 3 : 1;
+    const int r_hi = irc + 3 < 4 * nside - 1 ? irc + 3 : 4 * nside - 1;
+    double sum = 0.0;
+    for (int ir = r_lo; ir <= r_hi; ++ir) {
+        const RingEntry& e = rt[ir];
+        const int n = e.n;
+        const double step = e.dphi * e.sth;
+        int cnt = n <= 4 ? n : 5;
+        int ipc = 0;
+        if (n > 4) {
+            const int64_t fl = (int64_t)floor((phi - e.phi0) / e.dphi + 0.5);
+            ipc = (int)(((fl % n) + n) % n);
+        }
+        for (int j = 0; j < cnt; ++j) {
+            int ip = j;
+            if (n > 4) {
+                ip = ipc + j - 2;
+                ip = ip < 0 ? ip + n : (ip >= n ? ip - n : ip);
+            }
+            double g = e.phi0 + (double)ip * e.dphi - phi;  // longitude gap
+            g = g > kPi ? g - kTwoPi : (g < -kPi ? g + kTwoPi : g);
+            double sg, cg;
+            sincos_any(g, sg, cg);
+            const double d = fma(e.sth * st, cg, e.z * z);
+            if (!(d > 0.0 && step > 0.0 && dth > 0.0)) continue;
+            const double x = e.sth * sg / d / step;
+            const double y = fma(-z * e.sth, cg, e.z * st) / d / dth;
+            const double w = keys_kernel(x) * keys_kernel(y);
+            if (w != 0.0) {
+                f(ir, ip, w);
+                sum += w;
+            }
+        }
+    }
+    return sum;
 }
 
 }  // namespace todg

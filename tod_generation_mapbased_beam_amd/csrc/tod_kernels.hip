@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "healpix_ring.hpp"
+#include "internal.hpp"
 #include "../../include/todgpu.h"
 
 using namespace todg;
@@ -424,7 +425,9 @@ struct alignas(16) FpFrame {
     double lam;       // reference longitude
     double z_pts, sth_pts;  // boresight z and sin(theta)  (tod_bilinear.py:306-310)
     double sha, cha;  // sin, cos of half the boresight longitude minus lam
-    int ir_lo, nr, total, pad;
+    int ir_lo, nr, total;
+    int coinc_ring, coinc_pos;  // the map pixel whose centre IS the boresight, or -1
+    int pad;
 };
 
 // Map and geometry operands of the exact path (passed by value to the
@@ -443,7 +446,8 @@ __device__ __forceinline__ FpGlobals fp_globals(const TodArgs& a) {
 
 // Spin-2 inputs of one sample (uniform).
 struct FpSpin {
-    double z_pts, sth_pts, sha, cha, lam, bx, by;
+    double z_pts, sth_pts, sha, cha, lam;
+    int coinc_ring, coinc_pos;
 };
 
 // LDS layout of the footprint kernels (dynamic shared memory, compile-time
@@ -456,7 +460,9 @@ extern __shared__ __align__(16) unsigned char fp_smem[];
 
 constexpr int kFpCap = 4608;       // weight-grid cells
 constexpr int kFpMaxRings = 128;   // ring descriptors per buffer
-constexpr int kRingBuf = kFpMaxRings * (7 * 8 + 5 * 4);
+constexpr int kRingD = 12;        // double arrays per ring buffer
+constexpr int kRingI = 5;         // int arrays per ring buffer
+constexpr int kRingBuf = kFpMaxRings * (kRingD * 8 + kRingI * 4);
 constexpr int kOffRings = 0;
 constexpr int kOffFrame = kOffRings + 2 * kRingBuf;
 constexpr int kOffTrig = kOffFrame + 2 * (int)sizeof(FpFrame);
@@ -474,18 +480,26 @@ __device__ __forceinline__ T& lds_at(unsigned off) {
 // One buffer of footprint ring descriptors, structure of arrays.
 struct FpRings {
     unsigned o;  // byte offset of the buffer
-    __device__ double& base(int r) const { return lds_at<double>(o + 8 * r); }  // slot coordinate of lam, window-relative
-    __device__ double& inv_dphi(int r) const { return lds_at<double>(o + 8 * (kFpMaxRings + r)); }
-    __device__ double& z(int r) const { return lds_at<double>(o + 8 * (2 * kFpMaxRings + r)); }
-    __device__ double& sth(int r) const { return lds_at<double>(o + 8 * (3 * kFpMaxRings + r)); }
-    __device__ double& inv_dth(int r) const { return lds_at<double>(o + 8 * (4 * kFpMaxRings + r)); }
-    __device__ double& phi0(int r) const { return lds_at<double>(o + 8 * (5 * kFpMaxRings + r)); }
-    __device__ double& dphi(int r) const { return lds_at<double>(o + 8 * (6 * kFpMaxRings + r)); }
-    __device__ int& width(int r) const { return lds_at<int>(o + 56 * kFpMaxRings + 4 * r); }
-    __device__ int& off(int r) const { return lds_at<int>(o + 60 * kFpMaxRings + 4 * r); }
-    __device__ int& i0(int r) const { return lds_at<int>(o + 64 * kFpMaxRings + 4 * r); }
-    __device__ int& n(int r) const { return lds_at<int>(o + 68 * kFpMaxRings + 4 * r); }
-    __device__ int& first(int r) const { return lds_at<int>(o + 72 * kFpMaxRings + 4 * r); }  // npix < 2^31
+    __device__ double& d(int k, int r) const { return lds_at<double>(o + 8 * (k * kFpMaxRings + r)); }
+    __device__ int& i(int k, int r) const { return lds_at<int>(o + 8 * kRingD * kFpMaxRings + 4 * (k * kFpMaxRings + r)); }
+    __device__ double& base(int r) const { return d(0, r); }  // slot coordinate of lam, window-relative
+    __device__ double& inv_dphi(int r) const { return d(1, r); }
+    __device__ double& z(int r) const { return d(2, r); }
+    __device__ double& sth(int r) const { return d(3, r); }
+    __device__ double& inv_dth(int r) const { return d(4, r); }
+    __device__ double& phi0(int r) const { return d(5, r); }
+    __device__ double& dphi(int r) const { return d(6, r); }
+    // spin-2 constants of the ring against the sample's boresight (Spin2Ring)
+    __device__ double& st2(int r) const { return d(7, r); }
+    __device__ double& sS(int r) const { return d(8, r); }
+    __device__ double& s2dt(int r) const { return d(9, r); }
+    __device__ double& zsum(int r) const { return d(10, r); }
+    __device__ double& sc4(int r) const { return d(11, r); }
+    __device__ int& width(int r) const { return i(0, r); }
+    __device__ int& off(int r) const { return i(1, r); }    // non-decreasing in r
+    __device__ int& i0(int r) const { return i(2, r); }
+    __device__ int& n(int r) const { return i(3, r); }
+    __device__ int& first(int r) const { return i(4, r); }  // npix < 2^31
 };
 
 __device__ __forceinline__ FpRings fp_rings(int buf) { return FpRings{(unsigned)(kOffRings + buf * kRingBuf)}; }
@@ -522,35 +536,6 @@ __device__ __forceinline__ int fp_local(const FpRings& R, int r, int pos) {
     return d < R.width(r) ? R.off(r) + d : -1;
 }
 
-// sin and cos of x, |x| < 2 pi: quadrant reduction by a two-part pi/2 and
-// Taylor series on |r| <= pi/4 (truncation < 1e-19).
-__device__ __forceinline__ void sincos_red(double x, double& s, double& c) {
-    const double k = rint(x * (2.0 / kPi));
-    double r = fma(-k, 1.5707963267948966, x);
-    r = fma(-k, 6.123233995736766e-17, r);
-    const double r2 = r * r;
-    double ps = fma(r2, -1.0 / 355687428096000.0, 1.0 / 1307674368000.0);
-    ps = fma(r2, ps, -1.0 / 6227020800.0);
-    ps = fma(r2, ps, 1.0 / 39916800.0);
-    ps = fma(r2, ps, -1.0 / 362880.0);
-    ps = fma(r2, ps, 1.0 / 5040.0);
-    ps = fma(r2, ps, -1.0 / 120.0);
-    ps = fma(r2, ps, 1.0 / 6.0);
-    const double sr = fma(-(r * r2), ps, r);
-    double pc = fma(r2, 1.0 / 6402373705728000.0, -1.0 / 20922789888000.0);
-    pc = fma(r2, pc, 1.0 / 87178291200.0);
-    pc = fma(r2, pc, -1.0 / 479001600.0);
-    pc = fma(r2, pc, 1.0 / 3628800.0);
-    pc = fma(r2, pc, -1.0 / 40320.0);
-    pc = fma(r2, pc, 1.0 / 720.0);
-    pc = fma(r2, pc, -1.0 / 24.0);
-    pc = fma(r2, pc, 0.5);
-    const double cr = fma(-r2, pc, 1.0);
-    const int q = ((int)k) & 3;
-    s = q == 0 ? sr : (q == 1 ? cr : (q == 2 ? -sr : -cr));
-    c = q == 0 ? cr : (q == 1 ? -sr : (q == 2 ? -cr : sr));
-}
-
 // Exact boresight longitude (tod_bilinear.py:306-310), for the rare exact
 // coincidence test of the spin-2 guard.
 __device__ __noinline__ double boresight_phi(double bx, double by) {
@@ -559,25 +544,23 @@ __device__ __noinline__ double boresight_phi(double bx, double by) {
     return ph;
 }
 
-// cos 2delta, sin 2delta of one pixel centre (z_n, sth_n, phi_n) against the
-// boresight (tod_bilinear.py:84-145).  The longitude gap enters through half
-// angles: phi_pts - phi_n = (phi_pts - lam) - (phi_n - lam), with the
-// boresight half angle precomputed per sample, so no atan2 is needed.
-__device__ __forceinline__ void fp_spin2(const FpSpin& sp, double z_n, double sth_n, double phi_n,
-                                         double& c2d, double& s2d) {
-    if (z_n == sp.z_pts && sth_n == sp.sth_pts) {
-        if (boresight_phi(sp.bx, sp.by) == phi_n) {  // tod_bilinear.py:108-109
-            c2d = 1.0;
-            s2d = 0.0;
-            return;
-        }
+// cos 2delta, sin 2delta of map pixel (ring, pos) with centre (z_n, sth_n,
+// phi_n) against the boresight (tod_bilinear.py:84-145).  The longitude gap
+// enters through half angles: phi_pts - phi_n = (phi_pts - lam) - (phi_n -
+// lam), with the boresight half angle precomputed per sample, so no atan2 is
+// needed.  The reference's exact-coincidence guard (tod_bilinear.py:108-109)
+// compares with the one pixel, found per sample, whose centre equals the
+// boresight bit for bit.
+__device__ __forceinline__ void fp_spin2(const FpSpin& sp, int ring, int pos, double z_n,
+                                         double sth_n, double phi_n, double& c2d, double& s2d) {
+    if (ring == sp.coinc_ring && pos == sp.coinc_pos) {
+        c2d = 1.0;
+        s2d = 0.0;
+        return;
     }
     const double hb = (phi_n - sp.lam) * 0.5;
     double sb, cb;
-    if (fabs(hb) <= 0.3)
-        sincos_small(hb, sb, cb);
-    else
-        sincos_red(hb, sb, cb);
+    sincos_any(hb, sb, cb);
     const double sh = fma(sp.sha, cb, -(sp.cha * sb));
     const double ch = fma(sp.cha, cb, sp.sha * sb);
     spin2_from_half(spin2_ring(z_n, sth_n, sp.z_pts, sp.sth_pts), sh, ch, c2d, s2d);
@@ -593,7 +576,7 @@ __device__ __forceinline__ Acc4 fp_direct(const FpGlobals& a, const FpSpin& sp, 
     const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
     if constexpr (QU) {
         double c2d, s2d;
-        fp_spin2(sp, e.z, e.sth, e.phi0 + (double)pos * e.dphi, c2d, s2d);
+        fp_spin2(sp, ring, pos, e.z, e.sth, e.phi0 + (double)pos * e.dphi, c2d, s2d);
         const double qv = (double)m.x, uv = (double)m.y;
         r.v[0] = w * fma(qv, c2d, uv * s2d);
         r.v[1] = w * fma(uv, c2d, -(qv * s2d));
@@ -615,8 +598,8 @@ __device__ __forceinline__ FpSpin fp_spin_of(const FpFrame* fr) {
     sp.sha = uniform(fr->sha);
     sp.cha = uniform(fr->cha);
     sp.lam = uniform(fr->lam);
-    sp.bx = uniform(fr->q[6]);
-    sp.by = uniform(fr->q[7]);
+    sp.coinc_ring = __builtin_amdgcn_readfirstlane(fr->coinc_ring);
+    sp.coinc_pos = __builtin_amdgcn_readfirstlane(fr->coinc_pos);
     return sp;
 }
 
@@ -668,7 +651,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
                 for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
             }
         }
-    } else {
+    } else if constexpr (MODE == 1) {
         const double sin_th = sqrt(fmax(0.0, vx * vx + vy * vy));
         int ring, pos;
         nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, ring, pos);
@@ -678,6 +661,36 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
             atomicAdd(&W[li], bv);
         else
             acc = fp_direct<QU, CS>(a, sp, ring, pos, bv);
+    } else {
+        // bicubic judge: weights summed first, then scattered normalised
+        const double z = fmax(-1.0, fmin(1.0, vz));
+        const double sum = bicubic_visit(a.nside, a.rings, z, phi_w, [](int, int, double) {});
+        if (sum != 0.0) {
+            const double scale = bv / sum;
+            bicubic_visit(a.nside, a.rings, z, phi_w, [&](int ring, int pos, double w) {
+                const int lr = ring - ir_lo;
+                const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
+                if (li >= 0) {
+                    atomicAdd(&W[li], w * scale);
+                } else {
+                    const Acc4 d = fp_direct<QU, CS>(a, sp, ring, pos, w * scale);
+                    for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
+                }
+            });
+        } else {  // no candidate inside the support: bilinear (as the oracle)
+            Nb o;
+            ring_interp(a.nside, a.rings, z, phi_w, o);
+            for (int k = 0; k < 4; ++k) {
+                const int lr = o.ring[k] - ir_lo;
+                const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
+                if (li >= 0) {
+                    atomicAdd(&W[li], o.w[k] * bv);
+                } else {
+                    const Acc4 d = fp_direct<QU, CS>(a, sp, o.ring[k], o.pos[k], o.w[k] * bv);
+                    for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
+                }
+            }
+        }
     }
     return acc;
 }
@@ -702,7 +715,8 @@ __device__ __forceinline__ BeamSrc beam_src(const TodArgs& a) {
     return BeamSrc{a.vec_orig, a.axes, a.cos_a, a.sin_a, a.ax_pts, a.cos_p, a.sin_p, a.S, a.nside};
 }
 
-__device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* red) {
+__device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* red,
+                                                double margin_px) {
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -744,13 +758,13 @@ __device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* r
         cosr = fmax(-1.0, fmin(1.0, mn));
     }
     // Disc radius for the ring range and the per-ring longitude half-width:
-    // beam radius + 2 pixel sizes (bilinear neighbours lie within ~1.5 of the
-    // query) + a rounding allowance.
+    // beam radius + margin_px pixel sizes (bilinear neighbours lie within
+    // ~1.5 of the query, bicubic candidates within ~4) + a rounding allowance.
     BeamFrame f;
     f.cx = cx;
     f.cy = cy;
     f.cz = cz;
-    f.R = fmin(kPi, acos(cosr) + 2.0 / (double)a.nside + 1e-6);
+    f.R = fmin(kPi, acos(cosr) + margin_px / (double)a.nside + 1e-6);
     f.cosR = cos(f.R);
     return f;
 }
@@ -813,6 +827,9 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         fr->m[6 + lane] = mz;
     }
 
+    // boresight z and sin(theta) (tod_bilinear.py:306-310)
+    const double z_pts = fmax(-1.0, fmin(1.0, q.bz));
+    const double sth_pts = sqrt(fmax(0.0, 1.0 - q.bz * q.bz));
     int carry = 0, laid = 0;
     for (int r0 = 0; r0 < nr; r0 += 64) {
         const int r = r0 + lane;
@@ -845,6 +862,12 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
                 width = g.n;
                 base = tc >= (double)g.n ? tc - (double)g.n : (tc < 0.0 ? tc + (double)g.n : tc);
             }
+            const Spin2Ring sr = spin2_ring(g.z, g.sth, z_pts, sth_pts);
+            R.st2(r) = sr.st2;
+            R.sS(r) = sr.S;
+            R.s2dt(r) = sr.s2dt;
+            R.zsum(r) = sr.zsum;
+            R.sc4(r) = sr.SC4;
             R.base(r) = base;
             R.inv_dphi(r) = g.inv_dphi;
             R.z(r) = g.z;
@@ -867,7 +890,7 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         if (r < nr) {
             R.i0(r) = i0;
             R.width(r) = fits ? width : 0;  // beyond capacity: accumulated directly
-            R.off(r) = fits ? off : 0;
+            R.off(r) = off;
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
@@ -906,9 +929,27 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         fr->q[9] = q.cp;
         fr->q[10] = q.sp;
         fr->q[11] = q.omp;
+        // A map pixel centre can equal the boresight only on a ring whose z
+        // and sin(theta) equal the boresight's bit for bit (e.g. z = 0).
+        int cr = -1, cp = -1;
+        const int ic = ring_above(nside, z_pts);
+        for (int ir = ic - 1; ir <= ic + 1; ++ir) {
+            if (ir < 1 || ir > 4 * nside - 1) continue;
+            const RingEntry& g = a.rings[ir];
+            if (g.z == z_pts && g.sth == sth_pts) {
+                const double ph = boresight_phi(q.bx, q.by);
+                const double k = rint((ph - g.phi0) / g.dphi);
+                if (k >= 0.0 && k < (double)g.n && g.phi0 + k * g.dphi == ph) {
+                    cr = ir;
+                    cp = (int)k;
+                }
+            }
+        }
+        fr->coinc_ring = cr;
+        fr->coinc_pos = cp;
         fr->lam = lam;
-        fr->z_pts = fmax(-1.0, fmin(1.0, q.bz));
-        fr->sth_pts = sqrt(fmax(0.0, 1.0 - q.bz * q.bz));
+        fr->z_pts = z_pts;
+        fr->sth_pts = sth_pts;
         fr->sha = sha;
         fr->cha = cha;
         fr->ir_lo = ir_lo;
@@ -1066,27 +1107,52 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, cons
 
 // All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
 // gather per map pixel with nonzero weight; the cell is cleared for the next
-// sample.  Returns the per-thread sums (outside-grid sums included).
+// sample.  Each wave takes 64 consecutive grid cells at a time (one per
+// lane), which span one or a few rings; the spin-2 terms that depend only on
+// the ring were computed with the layout.  Returns the per-thread sums
+// (outside-grid sums included).
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
                                           double* W, Acc4 acc) {
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     const int total = __builtin_amdgcn_readfirstlane(fr->total);
+    const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const FpSpin sp = fp_spin_of(fr);
-    int r = 0;
-    for (int j = tid; j < total; j += kFpThreads) {
-        while (r + 1 < nr && j >= R.off(r) + R.width(r)) ++r;
+    int r0 = 0;  // ring of the chunk's first cell (uniform, non-decreasing)
+    for (int c0 = wave * 64; c0 < total; c0 += kFpThreads) {
+        while (r0 + 1 < nr && R.off(r0 + 1) <= c0) ++r0;
+        const int j = c0 + lane;
+        if (j >= total) continue;
         const double wsum = W[j];
         if (wsum == 0.0) continue;
         W[j] = 0.0;
+        int r = r0;
+        while (r + 1 < nr && R.off(r + 1) <= j) ++r;
         int pos = R.i0(r) + (j - R.off(r));
         const int n = R.n(r);
         if (pos >= n) pos -= n;
         const float4 mv = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
         if constexpr (QU) {
             double c2d, s2d;
-            fp_spin2(sp, R.z(r), R.sth(r), R.phi0(r) + (double)pos * R.dphi(r), c2d, s2d);
+            if (ir_lo + r == sp.coinc_ring && pos == sp.coinc_pos) {  // tod_bilinear.py:108-109
+                c2d = 1.0;
+                s2d = 0.0;
+            } else {
+                const double hb = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
+                double sb, cb;
+                sincos_any(hb, sb, cb);
+                const double sh = fma(sp.sha, cb, -(sp.cha * sb));
+                const double ch = fma(sp.cha, cb, sp.sha * sb);
+                Spin2Ring g;
+                g.st2 = R.st2(r);
+                g.S = R.sS(r);
+                g.s2dt = R.s2dt(r);
+                g.zsum = R.zsum(r);
+                g.SC4 = R.sc4(r);
+                spin2_from_half(g, sh, ch, c2d, s2d);
+            }
             const double qv = (double)mv.x, uv = (double)mv.y;
             acc.v[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc.v[0]);
             acc.v[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
@@ -1189,7 +1255,7 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
     if (bbeg + kx >= bend) return;  // uniform over the block
     fp_init(a.lam_trig);
-    const BeamFrame bf = fp_beam_frame(beam_src(a), bbeg + kx, fp_scratch());
+    const BeamFrame bf = fp_beam_frame(beam_src(a), bbeg + kx, fp_scratch(), MODE == 2 ? 5.0 : 2.0);
     int64_t b = bbeg + kx;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
@@ -1267,7 +1333,7 @@ struct BatchCursor {
 // loads its item into `a`, its beam frame into c.bf.  Returns false when
 // every queue is drained.
 __device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, TodArgs& a,
-                                           int64_t& b) {
+                                           int64_t& b, double margin_px) {
     const int tid = threadIdx.x;
     const int xcd = blockIdx.x & 7;
     const int64_t qlen = (ba.total + 7) / 8;
@@ -1314,7 +1380,7 @@ __device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, 
             continue;
         }
         if (!c.have_frame) {  // beam frame from the item's first finite sample seen here
-            c.bf = fp_beam_frame(beam_src(a), b, fp_scratch());
+            c.bf = fp_beam_frame(beam_src(a), b, fp_scratch(), margin_px);
             c.have_frame = true;
         }
         return true;
@@ -1336,7 +1402,8 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
     fp_init(a.lam_trig);
     BatchCursor c = {0, 0, 0, -1, false, {0.0, 0.0, 1.0, 0.0, 1.0}};
     int64_t b = 0;
-    if (!batch_next(ba, c, a, b)) return;
+    const double margin_px = MODE == 2 ? 5.0 : 2.0;
+    if (!batch_next(ba, c, a, b, margin_px)) return;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
         fp_layout(a, b, c.bf, fp_trig(), fp_frame(0), fp_rings(0));
@@ -1345,7 +1412,7 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
     int buf = 0;
     for (;;) {
         int64_t bn = 0;
-        const bool have_next = batch_next(ba, c, a, bn);
+        const bool have_next = batch_next(ba, c, a, bn, margin_px);
         fp_step<MODE, QU, CS>(a, have_next, bn, c.bf, buf);
         if (!have_next) break;
         buf ^= 1;
@@ -1395,6 +1462,42 @@ __global__ __launch_bounds__(256) void k_nearest_indices(const TodArgs a, int64_
     int ring, pos;
     nearest_pix(a.nside, a.rings, vz, sin_th, phi_w, ring, pos);
     pix[i] = a.rings[ring].first + pos;
+}
+
+// Point values of one map field at N directions (z, phi): mode 0 = the
+// reference's bilinear interpolation (numba_healpy.py:358-499), 1 = the
+// bicubic judge (bicubic_visit, normalised; bilinear where no candidate
+// falls inside the support).
+__global__ __launch_bounds__(256) void k_interp_points(int mode, int nside,
+                                                       const RingEntry* __restrict__ rt,
+                                                       const float* __restrict__ map, int64_t npix,
+                                                       int64_t N, const double* __restrict__ z,
+                                                       const double* __restrict__ phi,
+                                                       double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double zz = z[i], pp = phi[i];
+    double acc = 0.0;
+    double sum = 0.0;
+    if (mode == 1) {
+        sum = bicubic_visit(nside, rt, zz, pp, [&](int ring, int pos, double w) {
+            acc += w * (double)map[rt[ring].first + pos];
+        });
+    }
+    if (sum != 0.0) {
+        acc /= sum;
+    } else {
+        Nb o;
+        ring_interp(nside, rt, zz, pp, o);
+        acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            int64_t p = rt[o.ring[k]].first + o.pos[k];
+            p = p < npix ? p : npix - 1;
+            acc += o.w[k] * (double)map[p];
+        }
+    }
+    out[i] = acc;
 }
 
 // Field-major (C, npix) -> pixel-major (npix, CS) with slot k taken from
@@ -1522,8 +1625,10 @@ hipError_t launch_batch_mode(const BatchArgs& ba, bool qu, int CS, int variant, 
 
 template <int MODE, bool QU, int CS>
 hipError_t launch_cs(const TodArgs& a, int variant, hipStream_t st) {
-    if (variant == 1 || variant == 2 || variant == 4)
-        return launch_direct_cs<MODE, QU, CS>(a, variant, st);
+    if constexpr (MODE != 2) {  // the bicubic judge runs on the footprint kernel only
+        if (variant == 1 || variant == 2 || variant == 4)
+            return launch_direct_cs<MODE, QU, CS>(a, variant, st);
+    }
     if (variant == 8) return launch_fp_cs<MODE, QU, CS, 2>(a, st);
     return launch_fp_cs<MODE, QU, CS, 4>(a, st);
 }
@@ -1593,13 +1698,22 @@ int run_tod(int mode, const float* vec_orig, const float* axes, const float* cos
     if (rc) return rc;
     if (B == 0 || S == 0) return TODG_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipError_t e = mode == 0 ? launch_mode<0>(a, qu, CS, variant, st)
-                             : launch_mode<1>(a, qu, CS, variant, st);
+    hipError_t e = mode == 0   ? launch_mode<0>(a, qu, CS, variant, st)
+                   : mode == 1 ? launch_mode<1>(a, qu, CS, variant, st)
+                               : launch_mode<2>(a, qu, CS, variant, st);
     if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TODG_OK;
 }
 
 }  // namespace
+
+namespace todg_internal {
+
+int set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+int ring_table(int64_t nside, const RingEntry** out) { return get_ring_table(nside, out); }
+
+}  // namespace todg_internal
 
 // ================================================================ C-ABI
 
@@ -1661,11 +1775,40 @@ int todg_gather_accum_nearest(const float* vec_orig, const float* axes, const fl
                    beam_vals, B, S, tod, c_q, c_u, variant, stream);
 }
 
+int todg_gather_accum_bicubic(const float* vec_orig, const float* axes, const float* cos_a,
+                              const float* sin_a, const float* ax_pts, const float* cos_p,
+                              const float* sin_p, int64_t nside, const float* map_packed, int C,
+                              const float* beam_vals, int64_t B, int64_t S, double* tod,
+                              int c_q, int c_u, int variant, void* stream) {
+    return run_tod(2, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
+                   beam_vals, B, S, tod, c_q, c_u, variant, stream);
+}
+
+int todg_interp_points(int mode, int64_t nside, const float* map, int64_t N, const double* z,
+                       const double* phi, double* out, void* stream) {
+    if (mode != 0 && mode != 1) return fail(TODG_ERR_ARG, "mode must be 0 (bilinear) or 1 (bicubic)");
+    if (!is_pow2(nside) || nside > 8192)
+        return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
+    if (N < 0) return fail(TODG_ERR_ARG, "N must be non-negative");
+    if (N == 0) return TODG_OK;
+    if (!map || !z || !phi || !out) return fail(TODG_ERR_ARG, "null pointer argument");
+    const RingEntry* rings = nullptr;
+    int rc = get_ring_table(nside, &rings);
+    if (rc) return rc;
+    const int64_t blocks = (N + 255) / 256;
+    hipLaunchKernelGGL(k_interp_points, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), mode, (int)nside, rings, map,
+                       12 * nside * nside, N, z, phi, out);
+    TODG_HIP_CHECK(hipGetLastError());
+    return TODG_OK;
+}
+
 int todg_gather_accum_batch(int mode, const todg_work_item* items, int n_items,
                             int64_t total_samples, int64_t nside, const float* map_packed, int C,
                             int c_q, int c_u, int chunk, unsigned int* queue, int* item_status,
                             int variant, void* stream) {
-    if (mode != 0 && mode != 1) return fail(TODG_ERR_ARG, "mode must be 0 (bilinear) or 1 (nearest)");
+    if (mode < 0 || mode > 2)
+        return fail(TODG_ERR_ARG, "mode must be 0 (bilinear), 1 (nearest) or 2 (bicubic)");
     if (!is_pow2(nside) || nside > 8192)
         return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
     if (n_items < 0 || total_samples < 0) return fail(TODG_ERR_ARG, "negative sizes");
@@ -1700,8 +1843,9 @@ int todg_gather_accum_batch(int mode, const todg_work_item* items, int n_items,
     ba.chunk = chunk;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int CS = slots_stride(nslots);
-    hipError_t e = mode == 0 ? launch_batch_mode<0>(ba, qu, CS, variant, st)
-                             : launch_batch_mode<1>(ba, qu, CS, variant, st);
+    hipError_t e = mode == 0   ? launch_batch_mode<0>(ba, qu, CS, variant, st)
+                   : mode == 1 ? launch_batch_mode<1>(ba, qu, CS, variant, st)
+                               : launch_batch_mode<2>(ba, qu, CS, variant, st);
     if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TODG_OK;
 }

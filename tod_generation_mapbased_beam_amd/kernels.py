@@ -16,8 +16,9 @@ import torch
 from . import _lib
 from .device import device_map_for, stream_handle, to_dev
 
-# variant 0: footprint kernel (per-sample LDS weight grid, default);
-# 1, 2, 4: direct kernel with that many samples per wavefront.
+# variant 0: footprint kernel (per-sample LDS weight grid, 4 waves/SIMD,
+# default); 8: the same compiled for 2 waves/SIMD; 1, 2, 4: direct kernel
+# with that many samples per wavefront.
 DEFAULT_VARIANT = 0
 
 
@@ -72,6 +73,41 @@ def _gather_accum_nearest_jit(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p
     """Nearest-pixel + spin-2 beam TOD on the GPU (tod_nearest.py:21-158)."""
     return _run(_lib.lib().todg_gather_accum_nearest, vec_orig, axes, cos_a, sin_a, ax_pts,
                 cos_p, sin_p, nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, variant)
+
+
+def _gather_accum_bicubic_jit(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside,
+                              mp_stacked, beam_vals, B, S, tod, c_q=-1, c_u=-1,
+                              variant=DEFAULT_VARIANT):
+    """Bicubic-judge + spin-2 beam TOD on the GPU (same arguments as the
+    bilinear kernel; README.md:120 describes the reference's bicubic branch)."""
+    if int(variant) not in (0, 8):
+        variant = DEFAULT_VARIANT
+    return _run(_lib.lib().todg_gather_accum_bicubic, vec_orig, axes, cos_a, sin_a, ax_pts,
+                cos_p, sin_p, nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, variant)
+
+
+INTERP_MODES = {"bilinear": 0, "bicubic": 1}
+
+
+def interp_points(nside, field, z, phi, mode="bilinear"):
+    """Values of one RING-ordered map field (npix,) at directions (z, phi):
+    bilinear (numba_healpy.py:358-499) or the bicubic judge.  Returns a
+    float64 torch tensor on the device (numpy in, numpy out)."""
+    if mode not in INTERP_MODES:
+        raise ValueError(f"mode {mode!r} not available; use 'bilinear' or 'bicubic'")
+    host = isinstance(z, np.ndarray)
+    m = to_dev(field, torch.float32).reshape(-1)
+    if m.numel() != 12 * int(nside) ** 2:
+        raise ValueError("field must hold 12 * nside**2 values")
+    zz = to_dev(z, torch.float64).reshape(-1)
+    pp = to_dev(phi, torch.float64).reshape(-1)
+    if zz.numel() != pp.numel():
+        raise ValueError("z and phi differ in length")
+    out = torch.empty(zz.numel(), dtype=torch.float64, device=zz.device)
+    _lib.check(_lib.lib().todg_interp_points(INTERP_MODES[mode], int(nside), _lib.ptr(m),
+                                             zz.numel(), _lib.ptr(zz), _lib.ptr(pp), _lib.ptr(out),
+                                             stream_handle()))
+    return out.cpu().numpy() if host else out
 
 
 def bilinear_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, B, S, b0, nb):

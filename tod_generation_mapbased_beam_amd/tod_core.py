@@ -35,8 +35,9 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
     Args mirror tod_core.py:52-96: ``data`` needs ``vec_orig (S,3)``,
     ``beam_vals (S,)``, ``comp_indices`` and (production) ``mp_stacked
     (C, npix)`` float32 (numpy, torch, or a packed ``DeviceMap``).
-    ``interp_mode`` is ``"bilinear"`` or ``"nearest"``; anything else raises
-    ``ValueError`` as the reference does.
+    ``interp_mode`` is ``"bilinear"`` or ``"nearest"`` (tod_core.py:114-119),
+    or ``"bicubic"`` (the reference's bicubic branch, README.md:120, 317);
+    anything else raises ``ValueError`` as the reference does.
 
     When ``mp_stacked`` is absent the reference falls back to a split
     healpy pipeline with bilinear weights and no spin-2 (tod_core.py:169-180);
@@ -53,10 +54,9 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
     mp_stacked = data.get("mp_stacked")
 
     c_q, c_u = _qu_slots(comp_indices)
-    use_nearest = interp_mode == "nearest"
-    if interp_mode not in ("nearest", "bilinear"):
+    if interp_mode not in ("nearest", "bilinear", "bicubic"):
         raise ValueError(
-            f"interp_mode {interp_mode!r} not available; use 'bilinear' or 'nearest'"
+            f"interp_mode {interp_mode!r} not available; use 'bilinear', 'nearest' or 'bicubic'"
         )
 
     on_device = isinstance(rot_vecs, torch.Tensor)
@@ -68,13 +68,16 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
     if mp_stacked is None:
         mp_stacked = torch.stack([to_dev(mp[c], torch.float32) for c in comp_indices])
         c_q = c_u = -1
-        use_nearest = False
+        if interp_mode == "nearest":
+            interp_mode = "bilinear"
 
     if variant is None:  # the autotuner's choice for this workload shape, if any
-        variant = autotune.tuned_variant(int(nside), S, "nearest" if use_nearest else "bilinear",
-                                         C, c_q >= 0 and c_u >= 0, kernels.DEFAULT_VARIANT)
+        variant = autotune.tuned_variant(int(nside), S, interp_mode, C, c_q >= 0 and c_u >= 0,
+                                         kernels.DEFAULT_VARIANT)
     tod_arr = torch.zeros((C, B), dtype=torch.float64, device=device())
-    fn = kernels._gather_accum_nearest_jit if use_nearest else kernels._gather_accum_fused_jit
+    fn = {"nearest": kernels._gather_accum_nearest_jit,
+          "bilinear": kernels._gather_accum_fused_jit,
+          "bicubic": kernels._gather_accum_bicubic_jit}[interp_mode]
     fn(vec_orig, *params, nside, mp_stacked, beam_vals, B, S, tod_arr, c_q, c_u, variant=variant)
     out32 = tod_arr.to(torch.float32)
     if on_device:
