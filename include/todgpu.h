@@ -188,6 +188,21 @@ int todg_corr_loss_grad(const float *params, const float *X, const float *wbil,
 int todg_corr_adam(float *params, const float *grad, float *m, float *v, int step, float lr,
                    float beta1, float beta2, float eps, void *stream);
 
+/* Beam-convolution TOD with the corrector fused into the gather: arguments
+ * of todg_gather_accum_fused plus the corrector's parameters.  Each beam
+ * pixel's 4 x 4 neighbourhood is gathered once, Q/U rotated per neighbour by
+ * the spin-2 angle, and the MLP (BF16 MFMA, FP32 accumulation) run per
+ * field; where the corrector does not apply (pole neighbourhoods) or its
+ * correction exceeds `bound` (in units of the neighbourhood RMS) the plain
+ * bilinear weights are used.  stats (optional, 2 counters, accumulated):
+ * corrected and fallback beam-pixel samples. */
+int todg_gather_accum_learned(const float *vec_orig, const float *axes, const float *cos_a,
+                              const float *sin_a, const float *ax_pts, const float *cos_p,
+                              const float *sin_p, int64_t nside, const float *map_packed, int C,
+                              const float *beam_vals, int64_t B, int64_t S, double *tod,
+                              int c_q, int c_u, const float *params, float bound,
+                              unsigned long long *stats, void *stream);
+
 /* C (M x N) = alpha op(A) op(B) + beta C on BF16 MFMA (row-major float32
  * operands rounded to bfloat16; ta / tb transpose A / B). */
 int todg_corr_gemm_bf16(int M, int N, int K, const float *A, int lda, int ta, const float *B,

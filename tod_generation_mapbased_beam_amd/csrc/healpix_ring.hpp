@@ -399,6 +399,26 @@ __device__ __forceinline__ void nearest_pix(int nside, const RingEntry* __restri
 }
 
 
+// ------------------------------------------------------------ map gathers
+
+// One pixel of the packed pixel-major map (npix, CS).  The reference never
+// bounds-checks; an index can only leave [0, npix) through the 1-ulp
+// "tw == n" float-mod corner at the last ring, or from a non-finite
+// pointing.  The address is clamped (the unsigned compare also catches
+// negatives) so such cases read a valid pixel, never fault.
+template <int CS>
+__device__ __forceinline__ float4 load_px(const float* __restrict__ map, int64_t p, int64_t npix) {
+    p = (uint64_t)p < (uint64_t)npix ? p : ((p < 0) ? 0 : npix - 1);
+    if constexpr (CS == 4) {
+        return *(reinterpret_cast<const float4*>(map) + p);
+    } else if constexpr (CS == 2) {
+        float2 v = *reinterpret_cast<const float2*>(map + 2 * p);
+        return make_float4(v.x, v.y, 0.f, 0.f);
+    } else {
+        return make_float4(map[p], 0.f, 0.f, 0.f);
+    }
+}
+
 // ------------------------------------------------------------ bicubic judge
 
 // Keys cubic convolution kernel, a = -0.5 (Catmull-Rom), support |t| < 2.

@@ -276,23 +276,6 @@ __device__ __forceinline__ double wrap_phi(double y, double x) {
     return p;
 }
 
-template <int CS>
-__device__ __forceinline__ float4 load_px(const float* __restrict__ map, int64_t p, int64_t npix) {
-    // The reference never bounds-checks; an index can only leave [0, npix)
-    // through the 1-ulp "tw == n" float-mod corner at the last ring, or from
-    // a non-finite pointing.  Clamp the address (unsigned compare also
-    // catches negatives) so such cases read a valid pixel, never fault.
-    p = (uint64_t)p < (uint64_t)npix ? p : ((p < 0) ? 0 : npix - 1);
-    if constexpr (CS == 4) {
-        return *(reinterpret_cast<const float4*>(map) + p);
-    } else if constexpr (CS == 2) {
-        float2 v = *reinterpret_cast<const float2*>(map + 2 * p);
-        return make_float4(v.x, v.y, 0.f, 0.f);
-    } else {
-        return make_float4(map[p], 0.f, 0.f, 0.f);
-    }
-}
-
 __device__ __forceinline__ float slot(const float4& v, int k) {
     return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
 }
@@ -1516,6 +1499,7 @@ __global__ __launch_bounds__(256) void k_pack_map(const float* __restrict__ src,
 
 int slots_stride(int nslots) { return nslots <= 1 ? 1 : (nslots == 2 ? 2 : 4); }
 
+
 bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // Derive the packed-slot order and the tod row of each slot from (C, c_q, c_u)
@@ -1712,6 +1696,15 @@ namespace todg_internal {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 
 int ring_table(int64_t nside, const RingEntry** out) { return get_ring_table(nside, out); }
+
+int map_slots(int C, int c_q, int c_u, bool* qu, int perm[4], int row[4], int* nslots, int* cs) {
+    int rc = slot_layout(C, c_q, c_u, qu, perm, row, nslots);
+    if (rc) return rc;
+    *cs = slots_stride(*nslots);
+    return TODG_OK;
+}
+
+int n_cu() { return cu_count(); }
 
 }  // namespace todg_internal
 
