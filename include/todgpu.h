@@ -150,6 +150,15 @@ int todg_ring_table(int64_t nside, double *z, double *theta, double *phi0, doubl
                     int64_t *first, int64_t *n);
 
 
+/* Beam transfer function B_ell = sum_i w_i P_ell(x_i), ell = 0..lmax, by the
+ * Legendre three-term recurrence per pixel (replaces the loop of compute_bell,
+ * tod_utils.py:147-251; selection, normalisation and the cos-distance x_i
+ * stay on the host).  x, w: N float64; ws: todg_beam_bell_workspace(N, lmax)
+ * float64 of device scratch; bell: lmax + 1 float64 (device). */
+size_t todg_beam_bell_workspace(int64_t N, int lmax);
+int todg_beam_bell(const double *x, const double *w, int64_t N, int lmax, double *ws,
+                   double *bell, void *stream);
+
 /* ------------------------------------------------------------------------
  * Learned interpolation corrector (no counterpart in the reference: the
  * build's corrector of bilinear interpolation toward the bicubic judge).

@@ -23,7 +23,7 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_NAME = "libtodgpu.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ["tod_kernels.hip", "corrector.hip"]
+SOURCES = ["tod_kernels.hip", "corrector.hip", "beam_tools.hip"]
 HEADERS = ["healpix_ring.hpp", "internal.hpp"]
 
 HIPCC_FLAGS = [

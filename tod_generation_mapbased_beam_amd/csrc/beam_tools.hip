@@ -1,0 +1,101 @@
+// Beam transfer function on the GPU:
+//
+//   B_ell = sum_i w_i P_ell(x_i),   ell = 0 .. lmax
+//
+// with the Legendre three-term recurrence per beam pixel
+// (tod_utils.py:147-251: P_{l+1} = ((2l+1) x P_l - l P_{l-1}) / (l+1)).
+// Each thread carries one pixel's recurrence; every ell's partial sum over a
+// block is reduced with wavefront shuffles and LDS, 64 multipoles per
+// barrier; a second kernel adds the blocks' partials.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "internal.hpp"
+#include "../../include/todgpu.h"
+
+namespace todg_beam {
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+constexpr int kChunk = 64;  // multipoles per LDS round
+
+__global__ __launch_bounds__(kThreads) void k_bell_partial(const double* __restrict__ x,
+                                                           const double* __restrict__ w,
+                                                           int64_t N, int lmax,
+                                                           double* __restrict__ part) {
+    __shared__ double red[kWaves][kChunk];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t i = (int64_t)blockIdx.x * kThreads + tid;
+    const double xi = i < N ? x[i] : 0.0;
+    const double wi = i < N ? w[i] : 0.0;
+    double p2 = 1.0, p1 = xi;  // P_{l-1}, P_l as the loop advances
+    for (int l0 = 0; l0 <= lmax; l0 += kChunk) {
+        for (int k = 0; k < kChunk; ++k) {
+            const int l = l0 + k;
+            double p;
+            if (l == 0) {
+                p = 1.0;
+            } else if (l == 1) {
+                p = xi;
+            } else {
+                const double lm = (double)(l - 1);
+                p = ((2.0 * lm + 1.0) * xi * p1 - lm * p2) / (lm + 1.0);
+                p2 = p1;
+                p1 = p;
+            }
+            double v = wi * p;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[wave][k] = v;
+        }
+        __syncthreads();
+        if (tid < kChunk && l0 + tid <= lmax) {
+            double s = 0.0;
+            for (int wv = 0; wv < kWaves; ++wv) s += red[wv][tid];
+            part[(int64_t)blockIdx.x * (lmax + 1) + l0 + tid] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bell_sum(const double* __restrict__ part, int nblocks,
+                                                  int lmax, double* __restrict__ bell) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l > lmax) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * (lmax + 1) + l];
+    bell[l] = s;
+}
+
+}  // namespace todg_beam
+
+extern "C" {
+
+size_t todg_beam_bell_workspace(int64_t N, int lmax) {
+    const int64_t nb = (N + todg_beam::kThreads - 1) / todg_beam::kThreads;
+    return (size_t)(nb > 0 ? nb : 1) * (size_t)(lmax + 1);
+}
+
+int todg_beam_bell(const double* x, const double* w, int64_t N, int lmax, double* ws, double* bell,
+                   void* stream) {
+    if (N < 0 || lmax < 0) return todg_internal::set_error(TODG_ERR_ARG, "N and lmax must be >= 0");
+    if (!bell || (N > 0 && (!x || !w || !ws)))
+        return todg_internal::set_error(TODG_ERR_ARG, "null pointer argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (N == 0) {
+        hipError_t e = hipMemsetAsync(bell, 0, (size_t)(lmax + 1) * sizeof(double), st);
+        return e == hipSuccess ? TODG_OK : todg_internal::set_error(TODG_ERR_HIP, hipGetErrorString(e));
+    }
+    const int64_t nb = (N + todg_beam::kThreads - 1) / todg_beam::kThreads;
+    if (nb > (1 << 24)) return todg_internal::set_error(TODG_ERR_ARG, "too many beam pixels");
+    hipLaunchKernelGGL(todg_beam::k_bell_partial, dim3((unsigned)nb), dim3(todg_beam::kThreads), 0, st,
+                       x, w, N, lmax, ws);
+    hipLaunchKernelGGL(todg_beam::k_bell_sum, dim3((unsigned)((lmax + 256) / 256)), dim3(256), 0, st,
+                       ws, (int)nb, lmax, bell);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : todg_internal::set_error(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+}  // extern "C"
