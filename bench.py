@@ -39,6 +39,28 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_tod_fp_bilinear_nside2048_S6364_B1728000_counters.md")
+
+
+def measured_traffic(mode, nside, S, B, variant):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it covers this
+    exact workload; None otherwise."""
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None
+    kv = {}
+    for ln in lines:
+        if " = " in ln:
+            k, v = ln.split(" = ", 1)
+            kv[k.strip()] = v.strip()
+    want = f"{mode} nside {nside} S {S} B {B} variant {variant}"
+    if kv.get("workload") != want or "traffic_bytes_per_launch" not in kv:
+        return None
+    return float(kv["traffic_bytes_per_launch"])
+
+
 def build_workload(nside, nside_beam, fwhm, n_samples, fs, seed, device):
     import torch
 
@@ -204,7 +226,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.mode, args.nside, S, B, args.variant),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
         },
         "cpu_baseline": cpu,
     }
