@@ -162,14 +162,19 @@ int todg_beam_bell(const double *x, const double *w, int64_t N, int lmax, double
 /* ------------------------------------------------------------------------
  * Learned interpolation corrector (no counterpart in the reference: the
  * build's corrector of bilinear interpolation toward the bicubic judge).
- * An MLP 18 -> 64 (GELU) -> 64 (GELU) -> 16 with FP32 master parameters in
- * one flat array of TODG_CORR_NPARAMS floats: W1 (64 x 18), b1, W2 (64 x 64),
- * b2, W3 (16 x 64), b3 (row-major, output x input).  Matrix products run on
- * BF16 MFMA with FP32 accumulation.  All pointers are device pointers. */
-#define TODG_CORR_NPARAMS 6416
+ * An MLP 21 -> 64 (GELU) -> 64 (GELU) -> 16 with FP32 master parameters in
+ * one flat array of TODG_CORR_NPARAMS floats: W1 (64 x 21), b1, W2 (64 x 64),
+ * b2, W3 (16 x 64), b3 (row-major, output x input).  Inputs: the query's
+ * sub-pixel coordinates in its 4 x 4 ring neighbourhood (fa, v, fb, fm, fp,
+ * each minus 1/2: slot fractions in rings ira, ira+1, ira-1, ira+2 and the
+ * bilinear ring weight v) and the 16 neighbourhood values normalised by
+ * their mean and RMS; outputs: 16 weight corrections (their mean is
+ * removed).  Matrix products run on BF16 MFMA with FP32 accumulation.  All
+ * pointers are device pointers. */
+#define TODG_CORR_NPARAMS 6608
 
 /* Features of N query directions (z, phi: float64) on one RING-ordered map
- * field (npix float32): X (N x 18: fa - 1/2, v - 1/2, the 4 x 4
+ * field (npix float32): X (N x 21: fa, v, fb, fm, fp minus 1/2, the 4 x 4
  * neighbourhood normalised by its mean and RMS), wbil (N x 4: bilinear
  * weights of neighbourhood cells 5, 6, 9, 10), mu, sigma (N), pix (N x 16
  * pixel indices) and ok (N: 0 where the neighbourhood leaves the normal
@@ -198,13 +203,16 @@ int todg_corr_adam(float *params, const float *grad, float *m, float *v, int ste
                    float beta1, float beta2, float eps, void *stream);
 
 /* Beam-convolution TOD with the corrector fused into the gather: arguments
- * of todg_gather_accum_fused plus the corrector's parameters.  Each beam
- * pixel's 4 x 4 neighbourhood is gathered once, Q/U rotated per neighbour by
- * the spin-2 angle, and the MLP (BF16 MFMA, FP32 accumulation) run per
- * field; where the corrector does not apply (pole neighbourhoods) or its
- * correction exceeds `bound` (in units of the neighbourhood RMS) the plain
- * bilinear weights are used.  stats (optional, 2 counters, accumulated):
- * corrected and fallback beam-pixel samples. */
+ * of todg_gather_accum_fused plus the corrector's parameters.  Runs on the
+ * footprint kernel: per beam pixel the corrector reads the 4 x 4
+ * neighbourhood of the first field outside the Q/U pair (I; Q when the map
+ * holds only Q and U), runs the MLP (BF16 MFMA, FP32 accumulation) and
+ * scatters the corrected weights, which every field then uses (Q/U rotated
+ * per pixel by the reference's spin-2 angle, tod_bilinear.py:403-432).
+ * Where the corrector does not apply (the rings next to the poles) or its
+ * correction exceeds `bound` (in units of the neighbourhood RMS; INFINITY:
+ * no bound) the plain bilinear weights are used.  stats (optional, 2
+ * counters, accumulated): corrected and plain beam-pixel samples. */
 int todg_gather_accum_learned(const float *vec_orig, const float *axes, const float *cos_a,
                               const float *sin_a, const float *ax_pts, const float *cos_p,
                               const float *sin_p, int64_t nside, const float *map_packed, int C,

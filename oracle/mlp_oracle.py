@@ -3,18 +3,22 @@
 Only tests/ may import this module.  It restates, in float64 numpy, the
 corrector the GPU library trains (csrc/corrector.hip):
 
-    x  = [fa - 1/2, v - 1/2, xhat_0..15]                       (18)
+    x  = [fa - 1/2, v - 1/2, fb - 1/2, fm - 1/2, fp - 1/2, xhat_0..15]   (21)
     h1 = gelu(W1 x + b1), h2 = gelu(W2 h1 + b2), o = W3 h2 + b3
     w  = w_bilinear + o - mean(o)
     p  = sum_k w_k xhat_k,  loss = sum_rows (p - that)^2
 
-with gelu(z) = z * sigmoid(1.702 z).  ``bf16=True`` rounds every matrix
+with gelu(z) = z * s(z), s the quintic smoothstep 1/2 + (15/16) t - (5/8) t^3 +
+(3/16) t^5 of t = clamp(z / 2.64, -1, 1) (GELU's Gaussian CDF replaced by a
+polynomial gate; max |gelu(z) - z Phi(z)| = 0.019).  ``bf16=True`` rounds every matrix
 product operand to bfloat16 (round to nearest even) as the MFMA kernels do,
 so gradients can be compared with the GPU's to FP32-accumulation accuracy;
 ``bf16=False`` is the exact float64 model, pinned against central finite
 differences in tests/test_corrector_oracle.py.  The 4 x 4 neighbourhood
 features follow numba_healpy.py:443-480 for the bilinear cell (restated by
-oracle/tod_oracle.c) and take rings ira-1 .. ira+2 x slots j-1 .. j+2.
+oracle/tod_oracle.c) and take rings ira-1 .. ira+2 x slots j-1 .. j+2;
+fa, fb, fm, fp are the query's slot fractions in rings ira, ira+1, ira-1,
+ira+2 and v the bilinear ring weight.
 """
 
 from __future__ import annotations
@@ -23,11 +27,11 @@ import math
 
 import numpy as np
 
-N_IN, N_H, N_OUT = 18, 64, 16
+GEO = 5
+N_IN, N_H, N_OUT = GEO + 16, 64, 16
 SIZES = [("W1", (N_H, N_IN)), ("b1", (N_H,)), ("W2", (N_H, N_H)), ("b2", (N_H,)),
          ("W3", (N_OUT, N_H)), ("b3", (N_OUT,))]
 N_PARAMS = sum(int(np.prod(s)) for _, s in SIZES)
-GELU_A = 1.702
 
 
 def unpack(flat):
@@ -51,17 +55,20 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def _sig(z):
-    return 1.0 / (1.0 + np.exp(-z))
+GATE_C = 2.64
 
 
 def gelu(z):
-    return z * _sig(GELU_A * z)
+    t = np.clip(z / GATE_C, -1.0, 1.0)
+    return z * (0.5 + t * (15.0 / 16.0 + t * t * (-5.0 / 8.0 + t * t * (3.0 / 16.0))))
 
 
 def gelu_grad(z):
-    s = _sig(GELU_A * z)
-    return s + GELU_A * z * s * (1.0 - s)
+    t = np.clip(z / GATE_C, -1.0, 1.0)
+    s = 0.5 + t * (15.0 / 16.0 + t * t * (-5.0 / 8.0 + t * t * (3.0 / 16.0)))
+    ds = np.where(np.abs(z) < GATE_C,
+                  (15.0 / 16.0 - (15.0 / 8.0) * t * t + (15.0 / 16.0) * t ** 4) / GATE_C, 0.0)
+    return s + z * ds
 
 
 def _mm(a, b, rnd):
@@ -81,7 +88,7 @@ def forward(flat, X, bf16_ops=False):
 
 
 def head(O, X, wbil):
-    xh = X[:, 2:]
+    xh = X[:, GEO:]
     w = O - O.mean(axis=1, keepdims=True)
     w[:, 5] += wbil[:, 0]
     w[:, 6] += wbil[:, 1]
@@ -100,7 +107,7 @@ def loss_grad(flat, X, wbil, that, bf16_ops=False):
     pred = head(O, X, np.asarray(wbil, dtype=np.float64))
     r = pred - np.asarray(that, dtype=np.float64)
     loss = float(np.sum(r * r))
-    xh = X[:, 2:]
+    xh = X[:, GEO:]
     dO = (2.0 / N) * r[:, None] * (xh - xh.mean(axis=1, keepdims=True))
     g = {}
     g["W3"] = _mm(dO.T, a2, bf16_ops)
@@ -129,8 +136,9 @@ def init_params(seed):
 
 
 def stencil16(O, nside, z, phi):
-    """4 x 4 neighbourhood pixels, bilinear fractions and validity of one
-    query, from the restated ring geometry (oracle/tod_oracle.c)."""
+    """4 x 4 neighbourhood pixels and the sub-pixel coordinates (fa, fb, v,
+    fm, fp) of one query, from the restated ring geometry
+    (oracle/tod_oracle.c); None outside rings 2 .. 4 nside - 3."""
     z = max(-1.0, min(1.0, z))
     ira = O.ring_above(nside, z)
     if not (2 <= ira <= 4 * nside - 3):
@@ -141,6 +149,7 @@ def stencil16(O, nside, z, phi):
     fa = w[1] / wa if wa > 0 else 0.0
     fb = w[3] / v if v > 0 else 0.0
     p16 = []
+    fr = [0.0, 0.0]
     for r in range(4):
         ir = ira - 1 + r
         n, first, phi0, dphi = O.ring_info(nside, ir)
@@ -153,6 +162,7 @@ def stencil16(O, nside, z, phi):
             if tw != 0 and tw < 0:
                 tw += n
             j = int(tw)
+            fr[0 if r == 0 else 1] = tw - j
         for s in range(4):
             p16.append(first + ((j - 1 + s) % n))
-    return np.array(p16, dtype=np.int64), fa, fb, v
+    return np.array(p16, dtype=np.int64), fa, fb, v, fr[0], fr[1]

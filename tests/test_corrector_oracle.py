@@ -11,9 +11,9 @@ from tod_generation_mapbased_beam_amd import corrector
 
 def _batch(n, seed):
     rng = np.random.default_rng(seed)
-    X = rng.standard_normal((n, 18))
-    X[:, :2] = rng.uniform(-0.5, 0.5, (n, 2))
-    xh = X[:, 2:]
+    X = rng.standard_normal((n, M.N_IN))
+    X[:, :M.GEO] = rng.uniform(-0.5, 0.5, (n, M.GEO))
+    xh = X[:, M.GEO:]
     xh -= xh.mean(axis=1, keepdims=True)
     xh /= np.sqrt(np.mean(xh ** 2, axis=1, keepdims=True))
     w = rng.uniform(0.0, 1.0, (n, 4))
@@ -53,7 +53,7 @@ def test_weights_sum_to_one_and_constant_neighbourhood():
     corr = O - O.mean(axis=1, keepdims=True)
     np.testing.assert_allclose(corr.sum(axis=1), 0.0, atol=1e-12)
     X0 = X.copy()
-    X0[:, 2:] = 0.0  # constant neighbourhood: xhat = 0 -> prediction = mean
+    X0[:, M.GEO:] = 0.0  # constant neighbourhood: xhat = 0 -> prediction = mean
     np.testing.assert_allclose(M.head(M.forward(p, X0)[0], X0, w), 0.0, atol=0)
 
 

@@ -45,12 +45,12 @@ def test_gemm_bf16(torch, M_, N_, K_, ta, tb):
 
 def _batch(n, seed):
     rng = np.random.default_rng(seed)
-    X = rng.standard_normal((n, 18)).astype(np.float32)
-    X[:, :2] = rng.uniform(-0.5, 0.5, (n, 2))
-    xh = X[:, 2:].astype(np.float64)
+    X = rng.standard_normal((n, M.N_IN)).astype(np.float32)
+    X[:, :M.GEO] = rng.uniform(-0.5, 0.5, (n, M.GEO))
+    xh = X[:, M.GEO:].astype(np.float64)
     xh -= xh.mean(axis=1, keepdims=True)
     xh /= np.sqrt(np.mean(xh ** 2, axis=1, keepdims=True))
-    X[:, 2:] = xh
+    X[:, M.GEO:] = xh
     w = rng.uniform(0.0, 1.0, (n, 4))
     w = (w / w.sum(axis=1, keepdims=True)).astype(np.float32)
     t = (rng.standard_normal(n) * 0.3).astype(np.float32)
@@ -72,7 +72,7 @@ def test_loss_and_gradient_match_oracle(torch):
     d = np.abs(pred.cpu().numpy() - po)
     assert np.quantile(d, 0.99) <= 2e-4 and d.max() <= 1e-2
     gg = g.cpu().numpy().astype(np.float64)
-    for name_off, n in ((0, 64 * 18), (1152, 64), (1216, 4096), (5312, 64), (5376, 1024), (6400, 16)):
+    for name_off, n in ((0, 64 * 21), (1344, 64), (1408, 4096), (5504, 64), (5568, 1024), (6592, 16)):
         a, b = gg[name_off:name_off + n], go[name_off:name_off + n]
         assert np.max(np.abs(a - b)) <= 2e-3 * np.max(np.abs(b)) + 1e-9, name_off
 
@@ -115,12 +115,13 @@ def test_features_match_oracle_neighbourhood(torch, nside):
         assert (ref is not None) == bool(ok[i])
         if ref is None:
             continue
-        p16, fa, fb, v = ref
+        p16, fa, fb, v, fm, fp = ref
         assert np.array_equal(pix[i], p16)
         m = field[p16].astype(np.float64)
         sd = np.sqrt(np.mean((m - m.mean()) ** 2))
-        np.testing.assert_allclose(X[i, 2:], (m - m.mean()) / sd, atol=1e-5)
-        np.testing.assert_allclose(X[i, :2], [fa - 0.5, v - 0.5], atol=1e-6)
+        np.testing.assert_allclose(X[i, M.GEO:], (m - m.mean()) / sd, atol=1e-5)
+        np.testing.assert_allclose(X[i, :M.GEO], [fa - 0.5, v - 0.5, fb - 0.5, fm - 0.5, fp - 0.5],
+                                   atol=1e-6)
         np.testing.assert_allclose(wb[i], [(1 - v) * (1 - fa), (1 - v) * fa, v * (1 - fb), v * fb],
                                    atol=1e-6)
 
@@ -140,14 +141,29 @@ def test_training_beats_bilinear_on_held_out_sky(torch):
     assert got < 0.5 * bil
 
 
+def _mlp_infer(flat, X):
+    """The fused kernel's MLP on rows X: bf16 operands with b1 folded into W1
+    (so rounded to bf16 as well), FP32 b2, b3."""
+    p = M.unpack(flat)
+    Xa = np.concatenate([X, np.ones((X.shape[0], 1))], axis=1)
+    W1a = np.concatenate([p["W1"], p["b1"][:, None]], axis=1)
+    a1 = M.gelu(M.bf16(Xa) @ M.bf16(W1a).T)
+    a2 = M.gelu(M.bf16(a1) @ M.bf16(p["W2"]).T + p["b2"])
+    return M.bf16(a2) @ M.bf16(p["W3"]).T + p["b3"]
+
+
 def _learned_oracle(vec, params_rot, nside, mp, bv, c_q, c_u, flat, bound=np.inf):
-    """CPU restatement of todg_gather_accum_learned on the oracle's geometry."""
+    """CPU restatement of todg_gather_accum_learned on the oracle's geometry:
+    the corrector reads the first field outside the Q/U pair (else Q), and
+    its weights serve every field, Q/U rotated by spin-2 per pixel."""
     B = params_rot[1].shape[0]
     S = vec.shape[0]
     C = mp.shape[0]
     d = O.rotated_dirs(vec, *params_rot, 0, B).astype(np.float64)
     tod = np.zeros((C, B))
     qu = c_q >= 0 and c_u >= 0
+    others = [c for c in range(C) if not qu or c not in (c_q, c_u)]
+    drv = others[0] if others else c_q
     for b in range(B):
         bx, by, bz = (float(x) for x in params_rot[3][b])
         z_pts = max(-1.0, min(1.0, bz))
@@ -158,8 +174,20 @@ def _learned_oracle(vec, params_rot, nside, mp, bv, c_q, c_u, flat, bound=np.inf
             ph = math.atan2(d[b, s, 1], d[b, s, 0])
             ph = ph + 2 * math.pi if ph < 0 else ph
             st = M.stencil16(O, nside, z, ph)
-            pix4, w4, _, _ = O.ring_interp(nside, z, ph)
-            pix = st[0] if st is not None else pix4
+            pix, w = O.ring_interp(nside, z, ph)[:2]
+            if st is not None:
+                p16, fa, fb, v, fm, fp = st
+                m = mp[drv, p16].astype(np.float64)
+                mu = m.mean()
+                sd = math.sqrt(np.mean((m - mu) ** 2))
+                xh = (m - mu) / sd if sd > 1e-30 else np.zeros(16)
+                X = np.concatenate([[fa - 0.5, v - 0.5, fb - 0.5, fm - 0.5, fp - 0.5], xh])[None, :]
+                o = _mlp_infer(flat, X)[0]
+                dw = o - o.mean()
+                if abs(float(np.dot(dw, xh))) <= bound:
+                    w16 = dw.copy()
+                    w16[[5, 6, 9, 10]] += [(1 - v) * (1 - fa), (1 - v) * fa, v * (1 - fb), v * fb]
+                    pix, w = p16, w16
             vals = mp[:, pix].astype(np.float64).copy()
             if qu:
                 for k, p in enumerate(pix):
@@ -168,21 +196,7 @@ def _learned_oracle(vec, params_rot, nside, mp, bv, c_q, c_u, flat, bound=np.inf
                     q, u = vals[c_q, k], vals[c_u, k]
                     vals[c_q, k] = q * c2 + u * s2
                     vals[c_u, k] = -q * s2 + u * c2
-            for c in range(C):
-                if st is None:
-                    val = float(np.dot(w4, vals[c]))
-                else:
-                    _, fa, fb, v = st
-                    m = vals[c]
-                    mu = m.mean()
-                    sd = math.sqrt(np.mean((m - mu) ** 2))
-                    xh = (m - mu) / sd if sd > 1e-30 else np.zeros(16)
-                    X = np.concatenate([[fa - 0.5, v - 0.5], xh])[None, :]
-                    o = M.forward(flat, M.bf16(X), bf16_ops=True)[0][0]
-                    corr = float(np.dot(o - o.mean(), xh))
-                    bil = (1 - v) * ((1 - fa) * m[5] + fa * m[6]) + v * ((1 - fb) * m[9] + fb * m[10])
-                    val = bil + sd * corr if abs(corr) <= bound else bil
-                tod[c, b] += float(bv[s]) * val
+            tod[:, b] += float(bv[s]) * (vals @ np.asarray(w, dtype=np.float64))
     return tod
 
 
@@ -205,23 +219,64 @@ def test_learned_tod_with_zero_correction_is_bilinear(torch):
         assert rel_err(got, ref) <= 2e-6
 
 
-def test_learned_tod_matches_oracle_restatement(torch):
+@pytest.mark.parametrize("case", ["compact", "iqu", "iqu_bound", "i_only", "qu_only", "nside32"])
+def test_learned_tod_matches_oracle_restatement(torch, case):
+    """The fused footprint kernel against the CPU restatement (BF16 MLP): a
+    compact beam on a ring scan (the certified fast path), and beams spread
+    over the sphere (wide footprints, the exact path) for the channel
+    layouts, a tight bound (many beam pixels fall back to bilinear) and an
+    nside whose full-sphere footprint exceeds the LDS grid (direct
+    accumulation)."""
     from cases import random_rot_params, rel_err
+    from tod_generation_mapbased_beam_amd import inputs
+    from tod_generation_mapbased_beam_amd import tod_rotations as R
     from tod_generation_mapbased_beam_amd.corrector import gather_accum_learned
 
     rng = np.random.default_rng(22)
-    nside, B, S = 16, 6, 40
+    if case == "compact":
+        nside, B = 64, 24
+        vec, bv, _ = inputs.gaussian_beam(128, 120.0, 0.99)
+        S = vec.shape[0]
+        params = R.rotation_params_for_pointing(*inputs.ring_scan(B, 0.5))
+    else:
+        nside, B, S = (32, 4, 60) if case == "nside32" else (16, 6, 40)
+        vec, params = random_rot_params(B, S, rng)
+        bv = rng.uniform(0.1, 1.0, S).astype(np.float32)
+        bv /= bv.sum()
+    C = {"i_only": 1, "qu_only": 2}.get(case, 3)
+    qu = {"i_only": (-1, -1), "qu_only": (0, 1)}.get(case, (1, 2))
+    bound = 0.05 if case == "iqu_bound" else np.inf
+    mp = rng.standard_normal((C, 12 * nside * nside)).astype(np.float32)
+    flat = M.init_params(3) * 4.0  # a visibly nonzero correction
+    got = np.zeros((C, B))
+    gather_accum_learned(vec, *params, nside, mp, bv, B, S, got, *qu, params=flat, bound=bound)
+    ref = _learned_oracle(vec, params, nside, mp, bv, *qu, flat.astype(np.float64), bound)
+    plain = np.zeros((C, B))
+    O.gather_accum_fused(vec, *params, nside, mp, bv, B, S, plain, *qu)
+    assert rel_err(got, ref) <= 5e-3
+    if bound == np.inf:
+        assert rel_err(got, plain) > 1e-2  # the correction is really applied
+
+
+def test_learned_stats_and_bound(torch):
+    """Counters add up to B*S; a zero bound with a nonzero correction falls
+    back to plain bilinear everywhere (the result is the bilinear kernel's)."""
+    from cases import random_rot_params, rel_err
+    from tod_generation_mapbased_beam_amd.corrector import gather_accum_learned
+
+    rng = np.random.default_rng(23)
+    nside, B, S = 32, 30, 200
     vec, params = random_rot_params(B, S, rng)
     mp = rng.standard_normal((3, 12 * nside * nside)).astype(np.float32)
     bv = rng.uniform(0.1, 1.0, S).astype(np.float32)
     bv /= bv.sum()
-    flat = M.init_params(3) * 4.0  # a visibly nonzero correction
-    for qu, bound in (((1, 2), np.inf), ((-1, -1), 0.05)):
-        got = np.zeros((3, B))
-        gather_accum_learned(vec, *params, nside, mp, bv, B, S, got, *qu, params=flat, bound=bound)
-        ref = _learned_oracle(vec, params, nside, mp, bv, *qu, flat.astype(np.float64), bound)
-        plain = np.zeros((3, B))
-        O.gather_accum_fused(vec, *params, nside, mp, bv, B, S, plain, *qu)
-        assert rel_err(got, ref) <= 5e-3
-        if bound == np.inf:
-            assert rel_err(got, plain) > 1e-2  # the correction is really applied
+    flat = M.init_params(5) * 4.0
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    got = np.zeros((3, B))
+    gather_accum_learned(vec, *params, nside, mp, bv, B, S, got, 1, 2, params=flat, bound=0.0,
+                         stats=stats)
+    st = stats.cpu().numpy()
+    assert st.sum() == B * S
+    ref = np.zeros((3, B))
+    O.gather_accum_fused(vec, *params, nside, mp, bv, B, S, ref, 1, 2)
+    assert rel_err(got, ref) <= 2e-6

@@ -24,7 +24,7 @@ LIB_NAME = "libtodgpu.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
 SOURCES = ["tod_kernels.hip", "corrector.hip", "beam_tools.hip"]
-HEADERS = ["healpix_ring.hpp", "internal.hpp"]
+HEADERS = ["healpix_ring.hpp", "internal.hpp", "corrector.hpp"]
 
 HIPCC_FLAGS = [
     "-O3",

@@ -1,11 +1,12 @@
 """Learned interpolation corrector: training data, training loop and point
 prediction on the GPU (kernels in csrc/corrector.hip).
 
-The corrector is an MLP 18 -> 64 -> 64 -> 16 (GELU) that looks at the
-sub-pixel position of a direction inside its bilinear cell and at the 4 x 4
-map neighbourhood of that cell (normalised by its mean and RMS), and emits
-corrections to the 16 interpolation weights (mean removed, so constant maps
-stay exact).  It is trained on (bilinear input, bicubic target) pairs: the
+The corrector is an MLP 21 -> 64 -> 64 -> 16 (GELU) that looks at the
+sub-pixel position of a direction in the 4 x 4 ring neighbourhood of its
+bilinear cell (the slot fractions in the four rings and the bilinear ring
+weight) and at the neighbourhood's 16 map values (normalised by their mean
+and RMS), and emits corrections to the 16 interpolation weights (mean
+removed, so constant maps stay exact).  It is trained on (bilinear input, bicubic target) pairs: the
 target is the bicubic judge (Keys/Catmull-Rom weights on the gnomonic tangent
 plane, README.md:120 of the reference) at uniformly random directions on
 synthetic skies with the spectrum C_l^TT = 1e3/(l(l+1)+1).
@@ -26,8 +27,9 @@ import torch
 from . import _lib, inputs, kernels
 from .device import device, stream_handle, to_dev
 
-N_PARAMS = 6416
-N_IN, N_H, N_OUT = 18, 64, 16
+N_PARAMS = 6608
+GEO = 5
+N_IN, N_H, N_OUT = GEO + 16, 64, 16
 
 
 def init_params(seed: int) -> np.ndarray:
@@ -48,7 +50,7 @@ def init_params(seed: int) -> np.ndarray:
 
 @dataclass
 class Features:
-    X: torch.Tensor      # (N, 18) float32
+    X: torch.Tensor      # (N, 21) float32
     wbil: torch.Tensor   # (N, 4) float32
     mu: torch.Tensor     # (N,) float32
     sigma: torch.Tensor  # (N,) float32
@@ -146,12 +148,20 @@ class Corrector:
                                              _lib.ptr(self.v), self.step, lr, beta1, beta2, eps,
                                              stream_handle()))
 
-    def train(self, X, wbil, that, epochs=1, batch=8192, lr=1e-3, seed=0, log=None):
-        """Adam over shuffled mini-batches; returns the mean loss per epoch."""
+    def train(self, X, wbil, that, epochs=1, batch=8192, lr=1e-3, seed=0, log=None,
+              schedule="constant"):
+        """Adam over shuffled mini-batches; returns the mean loss per epoch.
+        schedule: "constant" (lr throughout) or "cosine" (lr decayed to zero
+        over all steps along a half cosine)."""
+        if schedule not in ("constant", "cosine"):
+            raise ValueError("schedule must be 'constant' or 'cosine'")
         N = int(X.shape[0])
         gen = torch.Generator(device=X.device)
         gen.manual_seed(int(seed))
+        steps_per_epoch = max(1, N // batch)
+        total = epochs * steps_per_epoch
         hist = []
+        k = 0
         for ep in range(epochs):
             perm = torch.randperm(N, generator=gen, device=X.device)
             tot = torch.zeros((), dtype=torch.float64, device=X.device)
@@ -159,8 +169,10 @@ class Corrector:
                 idx = perm[i0:i0 + batch]
                 loss, grad, _ = self.loss_grad(X[idx].contiguous(), wbil[idx].contiguous(),
                                                that[idx].contiguous())
-                self.adam(grad, lr=lr)
+                step_lr = lr if schedule == "constant" else 0.5 * lr * (1.0 + math.cos(math.pi * k / total))
+                self.adam(grad, lr=step_lr)
                 tot += loss[0]
+                k += 1
             mean = float(tot) / max(1, (N // batch) * batch)
             hist.append(mean)
             if log:
@@ -194,8 +206,12 @@ def gather_accum_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nsi
                          stats=None):
     """Beam TOD with the corrector fused into the gather (same arguments as
     ``kernels._gather_accum_fused_jit`` plus the corrector's parameters);
-    accumulates into ``tod``.  ``stats``: optional int64 device tensor (2,)
-    that receives (corrected, fallback) beam-pixel-sample counts."""
+    accumulates into ``tod``.  The corrector reads the first field outside
+    the Q/U pair (I) and its corrected weights serve every field (Q/U
+    rotated by spin-2 per pixel).  ``bound``: largest |correction| applied,
+    in units of the neighbourhood RMS (plain bilinear beyond it).
+    ``stats``: optional int64 device tensor (2,) that receives (corrected,
+    plain bilinear) beam-pixel-sample counts."""
     from .device import device_map_for
 
     B, S, c_q, c_u = int(B), int(S), int(c_q), int(c_u)
@@ -223,5 +239,6 @@ def gather_accum_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nsi
 
 def bilinear_loss(X, wbil, that) -> float:
     """Mean normalised squared residual of plain bilinear weights."""
-    p = (wbil[:, 0] * X[:, 7] + wbil[:, 1] * X[:, 8] + wbil[:, 2] * X[:, 11] + wbil[:, 3] * X[:, 12])
+    xh = X[:, GEO:]
+    p = wbil[:, 0] * xh[:, 5] + wbil[:, 1] * xh[:, 6] + wbil[:, 2] * xh[:, 9] + wbil[:, 3] * xh[:, 10]
     return float(torch.mean((p.double() - that.double()) ** 2))

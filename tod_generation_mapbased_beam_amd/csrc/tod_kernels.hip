@@ -39,6 +39,7 @@
 
 #include "healpix_ring.hpp"
 #include "internal.hpp"
+#include "corrector.hpp"
 #include "../../include/todgpu.h"
 
 using namespace todg;
@@ -210,6 +211,11 @@ struct TodArgs {
     int nside;
     int nslots;
     int row[kMaxSlots];      // tod row of each packed slot
+    // learned corrector (footprint mode 3 only)
+    const float* corr;                 // FP32 parameters (corrector.hpp layout)
+    unsigned long long* corr_stats;    // [0] corrected, [1] plain bilinear beam-pixel samples, or null
+    float corr_bound;                  // largest |correction| applied, in neighbourhood RMS units
+    int corr_slot;                     // packed slot whose values drive the corrector
 };
 
 struct Sample {
@@ -454,6 +460,12 @@ constexpr int kOffScratch = kOffRed + kFpWaves * kMaxSlots * 8;
 constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
 constexpr int kOffW = kOffJob + 2 * 64;
 constexpr int kFpSmemBytes = kOffW + kFpCap * 8;
+// mode 3 (learned corrector) adds the driving field's values of the
+// footprint cells and one 64 x 64-byte staging tile per wave
+constexpr int kOffIv = kFpSmemBytes;
+constexpr int kOffStage = kOffIv + kFpCap * 4;
+constexpr int kStageBytes = 64 * 64;
+constexpr int kFpSmemBytesCorr = kOffStage + (kFpWaves - 1) * kStageBytes;  // waves 1..7
 
 template <typename T>
 __device__ __forceinline__ T& lds_at(unsigned off) {
@@ -1088,6 +1100,475 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, cons
     return accd;
 }
 
+// ============================================ footprint mode 3: corrector
+
+// The corrector's matrices as MFMA A operands, held in registers for the
+// life of the kernel.  Each layer is computed transposed, out^T = W in^T,
+// with v_mfma_f32_16x16x32_bf16 (lane l: A[l%16][8(l/16) .. +7],
+// B[8(l/16) .. +7][l%16], D[4(l/16) .. +3][l%16]): a layer's output tile
+// leaves lane l holding features 4(l/16) .. +3 of row l%16, which is the B
+// operand of the next layer once the next layer's k order is permuted so
+// that k-block kb, lane group g holds features 16(2kb) + 4g .. +3 and
+// 16(2kb+1) + 4g .. +3.  Layer 1 reads its inputs through a per-wave LDS
+// tile in natural k order, with b1 folded in as input 21 = 1.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct CorrRegs {
+    bf16x8 w1[4];
+    bf16x8 w2[4][2];
+    bf16x8 w3[2];
+    f32x4 b2[4];
+    f32x4 b3;
+};
+
+__device__ __forceinline__ int corr_perm(int kb, int g, int j) {
+    return 16 * (2 * kb + (j >> 2)) + 4 * g + (j & 3);
+}
+
+__device__ __forceinline__ void corr_load(const float* __restrict__ P, CorrRegs& c) {
+    using namespace todg_corr;
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int ft = 0; ft < 4; ++ft) {
+        const int o = 16 * ft + r;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = 8 * g + j;
+            const float w = k < kIn ? P[kOffW1 + o * kIn + k] : (k == kIn ? P[kOffB1 + o] : 0.f);
+            c.w1[ft][j] = (__bf16)w;
+        }
+#pragma unroll
+        for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) c.w2[ft][kb][j] = (__bf16)P[kOffW2 + o * kH + corr_perm(kb, g, j)];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) c.b2[ft][q] = P[kOffB2 + 16 * ft + 4 * g + q];
+    }
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) c.w3[kb][j] = (__bf16)P[kOffW3 + r * kH + corr_perm(kb, g, j)];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c.b3[q] = P[kOffB3 + 4 * g + q];
+}
+
+__device__ __forceinline__ bf16x8 corr_act(const f32x4& lo, const f32x4& hi) {
+    bf16x8 b;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        b[q] = (__bf16)todg_corr::gelu(lo[q]);
+        b[4 + q] = (__bf16)todg_corr::gelu(hi[q]);
+    }
+    return b;
+}
+
+// The MLP over the wave's 64 rows.  On entry every lane has written its
+// input row (32 bf16: 21 inputs, the constant 1, zeros) to the wave's tile;
+// on return o[] holds the lane's 16 outputs (the tile is reused for them).
+__device__ __forceinline__ void corr_mlp(const CorrRegs& c, unsigned char* tile, float o[16]) {
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        unsigned char* row = tile + (16 * t + r) * 64 + 16 * g;
+        const bf16x8 x = *reinterpret_cast<const bf16x8*>(row);
+        const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+        f32x4 h[4];
+#pragma unroll
+        for (int ft = 0; ft < 4; ++ft) h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w1[ft], x, zero, 0, 0, 0);
+        const bf16x8 a0 = corr_act(h[0], h[1]);
+        const bf16x8 a1 = corr_act(h[2], h[3]);
+#pragma unroll
+        for (int ft = 0; ft < 4; ++ft) {
+            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][0], a0, c.b2[ft], 0, 0, 0);
+            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][1], a1, h[ft], 0, 0, 0);
+        }
+        const bf16x8 e0 = corr_act(h[0], h[1]);
+        const bf16x8 e1 = corr_act(h[2], h[3]);
+        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[0], e0, c.b3, 0, 0, 0);
+        out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[1], e1, out, 0, 0, 0);
+        *reinterpret_cast<f32x4*>(row) = out;  // row 16t + r, outputs 4g .. 4g + 3
+    }
+    const f32x4* mine = reinterpret_cast<const f32x4*>(tile + lane * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const f32x4 v = mine[q];
+        o[4 * q] = v[0];
+        o[4 * q + 1] = v[1];
+        o[4 * q + 2] = v[2];
+        o[4 * q + 3] = v[3];
+    }
+}
+
+// Writes one input row: geometry, normalised values, the constant 1.
+__device__ __forceinline__ void corr_put_row(unsigned char* tile, const float geo[5],
+                                             const float xh[16], bool on) {
+    bf16x8 v[4];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+        float x = 0.f;
+        if (k < 5) x = geo[k];
+        else if (k < 21) x = xh[k - 5];
+        else if (k == 21) x = 1.f;
+        v[k >> 3][k & 7] = (__bf16)(on ? x : 0.f);
+    }
+    bf16x8* dst = reinterpret_cast<bf16x8*>(tile + (threadIdx.x & 63) * 64);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) dst[q] = v[q];
+}
+
+__device__ __forceinline__ float* fp_Iv() { return &lds_at<float>(kOffIv); }
+
+// All threads: the driving field's value of every laid-out cell of a
+// layout (waves take rings, lanes take slots).
+template <int CS>
+__device__ __forceinline__ void fp_load_values(const TodArgs& a, const FpFrame* fr, const FpRings& R) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
+    float* Iv = fp_Iv();
+    for (int r = wave; r < nr; r += kFpWaves) {
+        const int w = R.width(r), off = R.off(r), i0 = R.i0(r), n = R.n(r);
+        const int64_t first = R.first(r);
+        for (int d = lane; d < w; d += 64) {
+            int pos = i0 + d;
+            if (pos >= n) pos -= n;
+            Iv[off + d] = slot(load_px<CS>(a.map, first + pos, a.npix), a.corr_slot);
+        }
+    }
+}
+
+// Corrector geometry of one beam pixel on the certified fast path (the same
+// index certification as fp_fast_bilinear, for all four rings of the 4 x 4
+// neighbourhood, which must lie inside the laid-out windows).  Returns
+// false, touching nothing, when the exact path must take it.
+struct CorrGeo {
+    int cell[4];          // grid cell of slot j_r - 1 in footprint ring la - 1 + r
+    double fa, fb, v;     // bilinear fractions and ring weight
+    float fm, fp;         // fractions in rings ira - 1, ira + 2
+};
+
+__device__ __forceinline__ bool fp_fast_corr(const double* __restrict__ m, int nside, int ir_lo,
+                                             int nr, const FpRings& R, double u0, double u1,
+                                             double u2, CorrGeo& G) {
+    const double xp = fma(m[0], u0, fma(m[1], u1, m[2] * u2));
+    const double yp = fma(m[3], u0, fma(m[4], u1, m[5] * u2));
+    const double zr = fma(m[6], u0, fma(m[7], u1, m[8] * u2));
+    const double z = fmax(-1.0, fmin(1.0, zr));
+    const double az = fabs(z);
+    double tr;
+    int ira;
+    bool ok;
+    if (az <= kTwoThirds) {
+        tr = fma(z, -1.5 * (double)nside, 2.0 * (double)nside);
+        ira = (int)tr;
+        const double fr = tr - (double)ira;
+        ok = (fr > kCertSlot) & (fr < 1.0 - kCertSlot);
+    } else {
+        tr = (double)nside * sqrt_fast(3.0 * (1.0 - az));
+        const int it = (int)tr;
+        ira = z < 0.0 ? 4 * nside - it - 1 : it;
+        const double fr = tr - (double)it;
+        ok = (fr > kCertPolar) & (fr < 1.0 - kCertPolar);
+    }
+    ok &= fabs(az - kTwoThirds) > 1e-12;
+    int la = ira - ir_lo;
+    ok &= (ira >= 2) & (ira <= 4 * nside - 3) & (la >= 1) & (la + 2 < nr);
+    ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
+    la = ok ? la : 1;
+    const double t = yp * rcp_nr(xp);
+    const double t2 = t * t;
+    double p = fma(t2, 1.0 / 13.0, -1.0 / 11.0);
+    p = fma(t2, p, 1.0 / 9.0);
+    p = fma(t2, p, -1.0 / 7.0);
+    p = fma(t2, p, 1.0 / 5.0);
+    p = fma(t2, p, -1.0 / 3.0);
+    const double dl = fma(t * t2, p, t);
+    double fr4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int rr = la - 1 + k;
+        const double tc = fma(dl, R.inv_dphi(rr), R.base(rr));
+        const int j = (int)tc;
+        const double f = tc - (double)j;
+        ok &= (tc >= 1.0) & (j + 2 < R.width(rr)) & (f > kCertSlot) & (f < 1.0 - kCertSlot);
+        fr4[k] = f;
+        G.cell[k] = R.off(rr) + j - 1;
+    }
+    const double sq = sqrt_fast((1.0 - z) * (1.0 + z));
+    const double x = fma(sq, R.z(la), -(z * R.sth(la)));
+    ok &= fabs(x) < 1e-2;
+    if (!ok) return false;
+    const double x2 = x * x;
+    double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
+    pp = fma(x2, pp, 3.0 / 40.0);
+    pp = fma(x2, pp, 1.0 / 6.0);
+    G.v = fma(x * x2, pp, x) * R.inv_dth(la);
+    G.fa = fr4[1];
+    G.fb = fr4[2];
+    G.fm = (float)fr4[0];
+    G.fp = (float)fr4[3];
+    return true;
+}
+
+// Exact-path corrector inputs of one beam pixel (the reference rotation,
+// atan2, ring_interp): geometry, the 16 values (from the value grid inside
+// the layout, from the map outside it) and whether the corrector applies.
+struct CorrIn {
+    float geo[5];
+    float val[16];
+    int applies;
+};
+
+__device__ __forceinline__ bool corr_neighbourhood(const FpGlobals& a, double z, double phi_w,
+                                                   Nb& o, double& fa, double& fb, double& v,
+                                                   double& fm, double& fp, int ring[4], int js[4]) {
+    const int ira = ring_above(a.nside, z);
+    if (!(ira >= 2 && ira <= 4 * a.nside - 3)) return false;
+    ring_interp(a.nside, a.rings, z, phi_w, o);
+    const double wa = o.w[0] + o.w[1];
+    v = o.w[2] + o.w[3];
+    fa = wa > 0.0 ? o.w[1] / wa : 0.0;
+    fb = v > 0.0 ? o.w[3] / v : 0.0;
+    fm = fp = 0.0;
+    for (int r = 0; r < 4; ++r) {
+        const RingEntry& e = a.rings[ira - 1 + r];
+        ring[r] = ira - 1 + r;
+        if (r == 1) {
+            js[r] = o.pos[0];
+        } else if (r == 2) {
+            js[r] = o.pos[2];
+        } else {
+            double fr;
+            js[r] = ring_slot(phi_w, e.phi0, e.dphi, e.inv_dphi, e.n, fr);
+            if (r == 0) fm = fr;
+            else fp = fr;
+        }
+    }
+    return true;
+}
+
+template <int CS>
+__device__ __noinline__ CorrIn fp_corr_exact_inputs(FpGlobals a, int buf, int cslot, float u0,
+                                                     float u1, float u2) {
+    const FpRings R = fp_rings(buf);
+    const FpFrame* fr = fp_frame(buf);
+    Sample q;
+    frame_sample(fr, q);
+    double vx, vy, vz;
+    rotate(q, (double)u0, (double)u1, (double)u2, vx, vy, vz);
+    const double phi_w = wrap_phi(vy, vx);
+    const double z = fmax(-1.0, fmin(1.0, vz));
+    CorrIn ci;
+    Nb o;
+    double fa, fb, v, fm, fp;
+    int ring[4], js[4];
+    ci.applies = corr_neighbourhood(a, z, phi_w, o, fa, fb, v, fm, fp, ring, js) ? 1 : 0;
+    for (int k = 0; k < 16; ++k) ci.val[k] = 0.f;
+    ci.geo[0] = (float)(fa - 0.5);
+    ci.geo[1] = (float)(v - 0.5);
+    ci.geo[2] = (float)(fb - 0.5);
+    ci.geo[3] = (float)(fm - 0.5);
+    ci.geo[4] = (float)(fp - 0.5);
+    if (!ci.applies) return ci;
+    const int ir_lo = fr->ir_lo, nr = fr->nr;
+    for (int r = 0; r < 4; ++r) {
+        const RingEntry& e = a.rings[ring[r]];
+        for (int t = 0; t < 4; ++t) {
+            int pos = js[r] - 1 + t;
+            pos = ((pos % e.n) + e.n) % e.n;
+            const int lr = ring[r] - ir_lo;
+            const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
+            ci.val[4 * r + t] = li >= 0 ? fp_Iv()[li]
+                                        : slot(load_px<CS>(a.map, e.first + pos, a.npix), cslot);
+        }
+    }
+    return ci;
+}
+
+// Per-beam-pixel output of the MLP handed to the exact deposit.
+struct CorrW {
+    float dw[16];  // o_k - mean(o)
+    int apply;
+};
+
+// Exact-path deposit of one beam pixel: corrected weights (bilinear +
+// correction) when `apply`, the reference's bilinear weights otherwise;
+// cells inside the layout go to the weight grid, the others are accumulated
+// directly.  Out of line: only beam pixels off the fast path come here.
+template <bool QU, int CS>
+__device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u0, float u1,
+                                                   float u2, float bvf, CorrW cw) {
+    if (!cw.apply) return fp_exact_pixel<0, QU, CS>(a, buf, u0, u1, u2, bvf);
+    const FpRings R = fp_rings(buf);
+    const FpFrame* fr = fp_frame(buf);
+    double* W = fp_W();
+    Sample q;
+    frame_sample(fr, q);
+    const FpSpin sp = fp_spin_of(fr);
+    double vx, vy, vz;
+    rotate(q, (double)u0, (double)u1, (double)u2, vx, vy, vz);
+    const double phi_w = wrap_phi(vy, vx);
+    const double z = fmax(-1.0, fmin(1.0, vz));
+    Nb o;
+    double fa, fb, v, fm, fp;
+    int ring[4], js[4];
+    Acc4 acc = {{0.0, 0.0, 0.0, 0.0}};
+    if (!corr_neighbourhood(a, z, phi_w, o, fa, fb, v, fm, fp, ring, js)) return acc;
+    const double bv = (double)bvf;
+    const int ir_lo = fr->ir_lo, nr = fr->nr;
+    for (int r = 0; r < 4; ++r) {
+        const RingEntry& e = a.rings[ring[r]];
+        for (int t = 0; t < 4; ++t) {
+            const int k = 4 * r + t;
+            double w = (double)cw.dw[k];
+            if (k == 5) w += o.w[0];
+            else if (k == 6) w += o.w[1];
+            else if (k == 9) w += o.w[2];
+            else if (k == 10) w += o.w[3];
+            w *= bv;
+            int pos = js[r] - 1 + t;
+            pos = ((pos % e.n) + e.n) % e.n;
+            const int lr = ring[r] - ir_lo;
+            const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
+            if (li >= 0) {
+                atomicAdd(&W[li], w);
+            } else {
+                const Acc4 d = fp_direct<QU, CS>(a, sp, ring[r], pos, w);
+                for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
+            }
+        }
+    }
+    return acc;
+}
+
+// Waves 1..7: phase 1 of one sample with the corrector.  Each wave takes 64
+// beam pixels at a time (one per lane): geometry and the 16 neighbourhood
+// values (fast path from the value grid, else the exact path), the MLP on
+// MFMA over the wave's 64 rows, then the corrected weights times the beam
+// value into the weight grid.  Where the corrector does not apply (the
+// rings next to the poles) or its correction exceeds the bound, the plain
+// bilinear weights are used.
+template <bool QU, int CS>
+__device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, const FpJob* J, const FpFrame* fr,
+                                               const FpRings& R, int buf, double* W,
+                                               const CorrRegs& C, unsigned long long& n_corr,
+                                               unsigned long long& n_fb) {
+    const int tid = threadIdx.x - 64;
+    const int wave = threadIdx.x >> 6;
+    const int nside = a.nside;
+    unsigned char* tile = fp_smem + kOffStage + (wave - 1) * kStageBytes;
+    const float* __restrict__ vec = uniform_ptr(J->vec_orig);
+    const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
+    const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
+    const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
+    const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
+    const float bound = a.corr_bound;
+    const float* Iv = fp_Iv();
+    double m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
+    Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};
+    for (int64_t s0 = 0; s0 < S; s0 += kP1Threads) {
+        const int64_t s = s0 + tid;
+        const bool live = s < S;
+        float u0 = 0.f, u1 = 0.f, u2 = 0.f, bvf = 0.f;
+        if (live) {
+            u0 = vec[3 * s];
+            u1 = vec[3 * s + 1];
+            u2 = vec[3 * s + 2];
+            bvf = bvs[s];
+        }
+        CorrGeo G;
+        int kind = 0;  // 0 no beam pixel, 1 fast path, 2 exact path (corrector applies), 3 exact, bilinear
+        float geo[5], val[16];
+        if (live && fp_fast_corr(m, nside, ir_lo, nr, R, (double)u0, (double)u1, (double)u2, G)) {
+            kind = 1;
+            geo[0] = (float)(G.fa - 0.5);
+            geo[1] = (float)(G.v - 0.5);
+            geo[2] = (float)(G.fb - 0.5);
+            geo[3] = G.fm - 0.5f;
+            geo[4] = G.fp - 0.5f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) val[4 * r + t] = Iv[G.cell[r] + t];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) geo[k] = 0.f;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) val[k] = 0.f;
+            if (live) {
+                const CorrIn ci = fp_corr_exact_inputs<CS>(fp_globals(a), buf, a.corr_slot, u0, u1, u2);
+                kind = ci.applies ? 2 : 3;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) geo[k] = ci.geo[k];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) val[k] = ci.val[k];
+            }
+        }
+        const bool on = kind == 1 || kind == 2;
+        float mean = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) mean += val[k];
+        mean *= 1.f / 16.f;
+        float var = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) var = fmaf(val[k] - mean, val[k] - mean, var);
+        const float sd = sqrtf(var * (1.f / 16.f));
+        const float inv = sd > 1e-30f ? 1.f / sd : 0.f;
+        float xh[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) xh[k] = (val[k] - mean) * inv;
+        corr_put_row(tile, geo, xh, on);
+        float o[16];
+        corr_mlp(C, tile, o);
+        float om = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) om += o[k];
+        om *= 1.f / 16.f;
+        float corr = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) corr = fmaf(o[k] - om, xh[k], corr);
+        const bool apply = on && fabsf(corr) <= bound;
+        if (live) {
+            if (apply) ++n_corr;
+            else ++n_fb;
+        }
+        if (kind == 1) {
+            const double bv = (double)bvf;
+            const double wb = G.v * bv;
+            const double wa = bv - wb;
+            double w16[16];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) w16[k] = apply ? (double)(o[k] - om) * bv : 0.0;
+            w16[5] += wa - wa * G.fa;
+            w16[6] += wa * G.fa;
+            w16[9] += wb - wb * G.fb;
+            w16[10] += wb * G.fb;
+            if (apply) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int t = 0; t < 4; ++t) atomicAdd(W + G.cell[r] + t, w16[4 * r + t]);
+            } else {
+                atomicAdd(W + G.cell[1] + 1, w16[5]);
+                atomicAdd(W + G.cell[1] + 2, w16[6]);
+                atomicAdd(W + G.cell[2] + 1, w16[9]);
+                atomicAdd(W + G.cell[2] + 2, w16[10]);
+            }
+        } else if (kind >= 2) {
+            CorrW cw;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) cw.dw[k] = o[k] - om;
+            cw.apply = apply ? 1 : 0;
+            const Acc4 d = fp_corr_exact_deposit<QU, CS>(fp_globals(a), buf, u0, u1, u2, bvf, cw);
+#pragma unroll
+            for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
+        }
+    }
+    return accd;
+}
+
 // All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
 // gather per map pixel with nonzero weight; the cell is cleared for the next
 // sample.  Each wave takes 64 consecutive grid cells at a time (one per
@@ -1196,7 +1677,9 @@ __device__ __forceinline__ void fp_set_job(const TodArgs& a, int64_t b, int buf)
 // layout's latency hides behind phase 1.
 template <int MODE, bool QU, int CS>
 __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64_t bn,
-                                        const BeamFrame& bfn, int buf) {
+                                        const BeamFrame& bfn, int buf, const CorrRegs* C = nullptr,
+                                        unsigned long long* n_corr = nullptr,
+                                        unsigned long long* n_fb = nullptr) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const FpRings R = fp_rings(buf);
@@ -1208,10 +1691,17 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
             fp_layout(an, bn, bfn, fp_trig(), fp_frame(buf ^ 1), fp_rings(buf ^ 1));
         }
     } else {
-        acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
+        if constexpr (MODE == 3)
+            acc = fp_phase1_corr<QU, CS>(an, fp_job(buf), fr, R, buf, fp_W(), *C, *n_corr, *n_fb);
+        else
+            acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
     }
     __syncthreads();
     acc = fp_phase2<QU, CS>(an, fr, R, fp_W(), acc);
+    // mode 3: the next sample's values go into the value grid, which no
+    // wave reads between the barrier above and the one below
+    if constexpr (MODE == 3)
+        if (have_next) fp_load_values<CS>(an, fp_frame(buf ^ 1), fp_rings(buf ^ 1));
 #pragma unroll
     for (int k = 0; k < kMaxSlots; ++k) {
         if (k < CS) {
@@ -1238,21 +1728,45 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
     if (bbeg + kx >= bend) return;  // uniform over the block
     fp_init(a.lam_trig);
-    const BeamFrame bf = fp_beam_frame(beam_src(a), bbeg + kx, fp_scratch(), MODE == 2 ? 5.0 : 2.0);
+    // footprint margin: bilinear neighbours lie within ~1.5 pixel sizes of
+    // the query, 4 x 4 corrector neighbourhoods within ~3.6, bicubic
+    // candidates within ~4
+    const double margin = MODE == 2 ? 5.0 : (MODE == 3 ? 4.5 : 2.0);
+    const BeamFrame bf = fp_beam_frame(beam_src(a), bbeg + kx, fp_scratch(), margin);
     int64_t b = bbeg + kx;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
         fp_layout(a, b, bf, fp_trig(), fp_frame(0), fp_rings(0));
     }
     __syncthreads();
+    CorrRegs C;
+    unsigned long long n_corr = 0, n_fb = 0;
+    if constexpr (MODE == 3) {
+        corr_load(a.corr, C);
+        fp_load_values<CS>(a, fp_frame(0), fp_rings(0));
+        __syncthreads();
+    }
     int buf = 0;
     for (;;) {
         const int64_t bn = b + per;
         const bool have_next = bn < bend;
-        fp_step<MODE, QU, CS>(a, have_next, bn, bf, buf);
+        fp_step<MODE, QU, CS>(a, have_next, bn, bf, buf, &C, &n_corr, &n_fb);
         if (!have_next) break;
         b = bn;
         buf ^= 1;
+    }
+    if constexpr (MODE == 3) {
+        if (a.corr_stats) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                n_corr += __shfl_xor(n_corr, off, 64);
+                n_fb += __shfl_xor(n_fb, off, 64);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                atomicAdd(&a.corr_stats[0], n_corr);
+                atomicAdd(&a.corr_stats[1], n_fb);
+            }
+        }
     }
 }
 
@@ -1557,7 +2071,7 @@ hipError_t launch_direct_cs(const TodArgs& a, int T, hipStream_t st) {
 // one workgroup per CU) or 4 (128 VGPRs, two workgroups per CU).
 template <int MODE, bool QU, int CS, int MINW>
 hipError_t launch_fp_cs(const TodArgs& a, hipStream_t st) {
-    const size_t smem = kFpSmemBytes;
+    const size_t smem = MODE == 3 ? kFpSmemBytesCorr : kFpSmemBytes;
     static std::once_flag once;
     std::call_once(once, [&] {
         (void)hipFuncSetAttribute((const void*)k_tod_fp<MODE, QU, CS, MINW>,
@@ -1609,12 +2123,17 @@ hipError_t launch_batch_mode(const BatchArgs& ba, bool qu, int CS, int variant, 
 
 template <int MODE, bool QU, int CS>
 hipError_t launch_cs(const TodArgs& a, int variant, hipStream_t st) {
-    if constexpr (MODE != 2) {  // the bicubic judge runs on the footprint kernel only
-        if (variant == 1 || variant == 2 || variant == 4)
-            return launch_direct_cs<MODE, QU, CS>(a, variant, st);
+    if constexpr (MODE == 3) {
+        // the corrector keeps its matrices in registers: the 2-waves build
+        return launch_fp_cs<MODE, QU, CS, 2>(a, st);
+    } else {
+        if constexpr (MODE < 2) {  // the bicubic judge runs on the footprint kernel only
+            if (variant == 1 || variant == 2 || variant == 4)
+                return launch_direct_cs<MODE, QU, CS>(a, variant, st);
+        }
+        if (variant == 8) return launch_fp_cs<MODE, QU, CS, 2>(a, st);
+        return launch_fp_cs<MODE, QU, CS, 4>(a, st);
     }
-    if (variant == 8) return launch_fp_cs<MODE, QU, CS, 2>(a, st);
-    return launch_fp_cs<MODE, QU, CS, 4>(a, st);
 }
 
 template <int MODE>
@@ -1666,6 +2185,10 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
     a->npix = 12 * nside * nside;
     a->nside = (int)nside;
     a->nslots = nslots;
+    a->corr = nullptr;
+    a->corr_stats = nullptr;
+    a->corr_bound = 0.f;
+    a->corr_slot = (*qu && nslots > 2) ? 2 : 0;  // the first field outside the Q/U pair
     *CS = slots_stride(nslots);
     return TODG_OK;
 }
@@ -1685,6 +2208,28 @@ int run_tod(int mode, const float* vec_orig, const float* axes, const float* cos
     hipError_t e = mode == 0   ? launch_mode<0>(a, qu, CS, variant, st)
                    : mode == 1 ? launch_mode<1>(a, qu, CS, variant, st)
                                : launch_mode<2>(a, qu, CS, variant, st);
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TODG_OK;
+}
+
+int run_learned(const float* vec_orig, const float* axes, const float* cos_a, const float* sin_a,
+                const float* ax_pts, const float* cos_p, const float* sin_p, int64_t nside,
+                const float* map_packed, int C, const float* beam_vals, int64_t B, int64_t S,
+                double* tod, int c_q, int c_u, const float* params, float bound,
+                unsigned long long* stats, void* stream) {
+    TodArgs a;
+    bool qu;
+    int CS;
+    int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
+                       beam_vals, B, S, tod, c_q, c_u, true, &a, &qu, &CS);
+    if (rc) return rc;
+    if (!(bound >= 0.f)) return fail(TODG_ERR_ARG, "bound must be >= 0 (inf: no bound)");
+    if (B == 0 || S == 0) return TODG_OK;
+    if (!params) return fail(TODG_ERR_ARG, "null corrector parameters");
+    a.corr = params;
+    a.corr_stats = stats;
+    a.corr_bound = bound;
+    hipError_t e = launch_mode<3>(a, qu, CS, 0, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TODG_OK;
 }
@@ -1775,6 +2320,16 @@ int todg_gather_accum_bicubic(const float* vec_orig, const float* axes, const fl
                               int c_q, int c_u, int variant, void* stream) {
     return run_tod(2, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
                    beam_vals, B, S, tod, c_q, c_u, variant, stream);
+}
+
+int todg_gather_accum_learned(const float* vec_orig, const float* axes, const float* cos_a,
+                              const float* sin_a, const float* ax_pts, const float* cos_p,
+                              const float* sin_p, int64_t nside, const float* map_packed, int C,
+                              const float* beam_vals, int64_t B, int64_t S, double* tod, int c_q,
+                              int c_u, const float* params, float bound,
+                              unsigned long long* stats, void* stream) {
+    return run_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
+                       beam_vals, B, S, tod, c_q, c_u, params, bound, stats, stream);
 }
 
 int todg_interp_points(int mode, int64_t nside, const float* map, int64_t N, const double* z,

@@ -8,9 +8,12 @@ BF16-MFMA training kernels with FP32 master weights and Adam.
 Evaluation (``evaluate``): beam TODs of several detectors on a synthetic
 nside-2048 sky with the plain bilinear kernel, the fused corrector and the
 bicubic judge; RMS and max residuals against the judge, kernel times, and
-the rotation-stability spread of each method (the spread of the TOD of one
-pointing under random rolls of a circular beam, which is zero for a perfect
-interpolator -- the metric of the reference's docs/beam_interpolation_accuracy.rst:70-76).
+two rotation-stability spreads per method, both zero for a perfect
+interpolator: (1) the same sky under global rotations about the polar axis
+by random angles alpha (a_lm -> a_lm exp(-i m alpha), pointings phi -> phi +
+alpha), which moves every pointing's sub-pixel position; (2) the TOD of one
+pointing under random rolls of a circular beam (the metric of the
+reference's docs/beam_interpolation_accuracy.rst:70-76).
 
     python -m tod_generation_mapbased_beam_amd.learned_eval --out results.json [--quick]
 """
@@ -36,8 +39,8 @@ def log(msg):
     print(f"[learned] {msg}", file=sys.stderr, flush=True)
 
 
-def train(nside=512, n_maps=8, n_pairs=10_000_000, n_held=1_000_000, epochs=3, batch=8192,
-          lr=1e-3, seed=0):
+def train(nside=512, n_maps=8, n_pairs=10_000_000, n_held=1_000_000, epochs=8, batch=4096,
+          lr=1e-3, seed=0, schedule="cosine"):
     t0 = time.time()
     X, W, T = C.make_pairs(nside, seeds=range(1, n_maps + 1), n_per_map=n_pairs // n_maps,
                            dir_seed=seed)
@@ -47,14 +50,15 @@ def train(nside=512, n_maps=8, n_pairs=10_000_000, n_held=1_000_000, epochs=3, b
     c = C.Corrector(seed=seed)
     torch.cuda.synchronize()
     t1 = time.time()
-    hist = c.train(X, W, T, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    hist = c.train(X, W, T, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log, schedule=schedule)
     torch.cuda.synchronize()
     t_train = time.time() - t1
     bil = C.bilinear_loss(Xh, Wh, Th)
     got = c.eval_loss(Xh, Wh, Th)
     res = {
         "nside": nside, "maps": n_maps, "train_pairs": int(X.shape[0]), "held_out_pairs": int(Xh.shape[0]),
-        "epochs": epochs, "batch": batch, "lr": lr, "train_seconds": t_train, "data_seconds": t_data,
+        "epochs": epochs, "batch": batch, "lr": lr, "schedule": schedule, "train_seconds": t_train,
+        "data_seconds": t_data,
         "loss_per_epoch": hist,
         "held_out_ms_residual_bilinear": bil, "held_out_ms_residual_corrector": got,
         "held_out_rms_ratio": math.sqrt(got / bil),
@@ -143,7 +147,10 @@ def evaluate(c: C.Corrector, nside=2048, n_det=8, n_samples=1_728_000, fwhm=30.0
     out["time_ratio_learned_over_bilinear"] = times["learned"] / times["bilinear"]
     st = stats.cpu().numpy()
     out["learned_corrected_fraction"] = float(st[0] / max(1, st[0] + st[1]))
-    out["rotation_stability"] = rotation_stability(c, dmap, vec, bv, nside, n_rot, n_rot_points, bound)
+    out["rotation_stability_roll"] = rotation_stability(c, dmap, vec, bv, nside, n_rot, n_rot_points, bound)
+    del dmap, mp
+    out["rotation_stability_global"] = global_rotation_stability(c, nside, lmax, vec, bv, bound=bound,
+                                                                 seed=seed)
     return out
 
 
@@ -171,6 +178,52 @@ def rotation_stability(c, dmap, vec, bv, nside, n_rot=32, n_points=64, bound=4.0
             C.gather_accum_learned(v, *params, nside, dmap, b, B, S, t, 1, 2, params=c.params, bound=bound)
         T = t[0].reshape(n_points, n_rot)  # the intensity channel
         out[k] = float(torch.sqrt(torch.mean(torch.var(T, dim=1, unbiased=False))))
+    return out
+
+
+def global_rotation_stability(c, nside, lmax, vec, bv, n_rot=3, n_samples=200_000, bound=4.0,
+                              seed=7):
+    """The intensity sky rotated about the polar axis by alpha = 0 and n_rot
+    random angles (a_lm -> a_lm exp(-i m alpha)), scanned with pointings
+    phi -> phi + alpha: every method's T-TOD should not change.  Returns per
+    method the RMS over samples of the deviation from the mean over alpha,
+    and that RMS relative to the T signal RMS."""
+    dev = device()
+    tt, _, _ = inputs.cl_spectrum(lmax)
+    rng = np.random.default_rng(seed)
+    alm = inputs._alm_realisation(tt, lmax, rng)
+    alphas = [0.0] + list(np.random.default_rng(seed + 1).uniform(0, 2 * math.pi, n_rot))
+    th, ph, ps = inputs.ring_scan(n_samples, 20.0)
+    S = vec.shape[0]
+    v = torch.as_tensor(vec, device=dev)
+    b = torch.as_tensor(bv, device=dev)
+    mm = np.arange(lmax + 1)
+    tods = {"bilinear": [], "learned": [], "bicubic": []}
+    for a in alphas:
+        t0 = time.time()
+        rot = alm * np.exp(-1j * mm * a)[None, :]
+        mp = inputs.synthesize_map(nside, [rot], lmax, device=dev).astype(np.float32)
+        dmap = DeviceMap(mp, -1, -1)
+        params = [torch.as_tensor(p, device=dev)
+                  for p in R.rotation_params_for_pointing(th, np.mod(ph + a, 2 * math.pi), ps)]
+        for k in tods:
+            t = torch.zeros((1, n_samples), dtype=torch.float64, device=dev)
+            if k == "bilinear":
+                kernels._gather_accum_fused_jit(v, *params, nside, dmap, b, n_samples, S, t)
+            elif k == "bicubic":
+                kernels._gather_accum_bicubic_jit(v, *params, nside, dmap, b, n_samples, S, t)
+            else:
+                C.gather_accum_learned(v, *params, nside, dmap, b, n_samples, S, t, params=c.params,
+                                       bound=bound)
+            tods[k].append(t[0])
+        del dmap, mp
+        log(f"global rotation alpha={a:.4f} ({time.time() - t0:.1f}s)")
+    out = {"alphas": [float(a) for a in alphas], "samples": n_samples}
+    for k, ts in tods.items():
+        T = torch.stack(ts)
+        dev_rms = float(torch.sqrt(torch.mean((T - T.mean(dim=0, keepdim=True)) ** 2)))
+        sig = float(torch.sqrt(torch.mean(T[0] ** 2)))
+        out[k] = {"rms_spread": dev_rms, "rms_spread_over_signal": dev_rms / sig}
     return out
 
 
