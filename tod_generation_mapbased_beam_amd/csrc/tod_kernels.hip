@@ -461,11 +461,16 @@ constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
 constexpr int kOffW = kOffJob + 2 * 64;
 constexpr int kFpSmemBytes = kOffW + kFpCap * 8;
 // mode 3 (learned corrector) adds the driving field's values of the
-// footprint cells and one 64 x 64-byte staging tile per wave
+// footprint cells, one 64 x 64-byte staging tile per wave and the two
+// per-buffer chunk counters of phase 1
 constexpr int kOffIv = kFpSmemBytes;
 constexpr int kOffStage = kOffIv + kFpCap * 4;
 constexpr int kStageBytes = 64 * 64;
-constexpr int kFpSmemBytesCorr = kOffStage + (kFpWaves - 1) * kStageBytes;  // waves 1..7
+constexpr int kOffCnt = kOffStage + kFpWaves * kStageBytes;
+constexpr int kOffBias = kOffCnt + 16;                   // b2 (64), b3 (16) as float
+constexpr int kFpSmemBytesCorr = kOffBias + 80 * 4;
+static_assert(kOffStage % 16 == 0 && kOffBias % 16 == 0, "16-byte LDS vectors");
+static_assert(kFpSmemBytesCorr <= 160 * 1024, "one workgroup per CU");
 
 template <typename T>
 __device__ __forceinline__ T& lds_at(unsigned off) {
@@ -1043,6 +1048,96 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     return true;
 }
 
+// cos x for |x| <= 0.05 by Taylor series (truncation x^10/10! < 3e-20).
+__device__ __forceinline__ double cos_tiny(double x) {
+    const double x2 = x * x;
+    double p = fma(x2, 1.0 / 40320.0, -1.0 / 720.0);
+    p = fma(x2, p, 1.0 / 24.0);
+    p = fma(x2, p, -0.5);
+    return fma(x2, p, 1.0);
+}
+
+// Nearest fast path for one beam pixel: the reference's candidate set
+// (rings ir, ir+1 x slots int(phi n / 2 pi), +1; tod_nearest.py:115-141)
+// with the candidates' cos-distances evaluated from the composite rotation.
+// Certification: the ring and both candidate slot coordinates are taken
+// only with fractional parts in [kCertSlot, 1 - kCertSlot] (kCertPolar for
+// the polar ring formula), so the candidates are the reference's; the
+// winner is taken only when its cos-distance exceeds every other
+// candidate's by more than kCertNearest, far above the few-ulp differences
+// between these cos-distances and the reference's, so it is the reference's
+// winner (ties, where the reference keeps the first candidate, fall to the
+// exact path).  The winner's cell gets the beam value.
+constexpr double kCertNearest = 1e-12;
+
+__device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, int nside, int ir_lo,
+                                                int nr, const FpRings& R, double* W, double u0,
+                                                double u1, double u2, double bv) {
+    const double xp = fma(m[0], u0, fma(m[1], u1, m[2] * u2));
+    const double yp = fma(m[3], u0, fma(m[4], u1, m[5] * u2));
+    const double z = fma(m[6], u0, fma(m[7], u1, m[8] * u2));  // not clamped (tod_nearest.py:112)
+    const double az = fabs(z);
+    double tr;
+    int ira;
+    bool ok;
+    if (az <= kTwoThirds) {
+        tr = fma(z, -1.5 * (double)nside, 2.0 * (double)nside);
+        ira = (int)tr;
+        const double fr = tr - (double)ira;
+        ok = (fr > kCertSlot) & (fr < 1.0 - kCertSlot);
+    } else {
+        tr = (double)nside * sqrt_fast(fmax(3.0 * (1.0 - az), 1e-300));
+        const int it = (int)tr;
+        ira = z < 0.0 ? 4 * nside - it - 1 : it;
+        const double fr = tr - (double)it;
+        ok = (fr > kCertPolar) & (fr < 1.0 - kCertPolar) & (az < 1.0);
+    }
+    ok &= fabs(az - kTwoThirds) > 1e-12;
+    int la = ira - ir_lo;
+    ok &= (ira >= 1) & (ira <= 4 * nside - 2) & (la >= 0) & (la + 1 < nr);
+    ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
+    la = ok ? la : 0;
+    const double t = yp * rcp_nr(xp);
+    const double t2 = t * t;
+    double p = fma(t2, 1.0 / 13.0, -1.0 / 11.0);
+    p = fma(t2, p, 1.0 / 9.0);
+    p = fma(t2, p, -1.0 / 7.0);
+    p = fma(t2, p, 1.0 / 5.0);
+    p = fma(t2, p, -1.0 / 3.0);
+    const double dl = fma(t * t2, p, t);
+    const double sth = sqrt_fast(fmax(fma(xp, xp, yp * yp), 1e-300));
+    double best = -2.0, second = -2.0;
+    int cell = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int r = la + k;
+        // slot coordinate of phi n / 2 pi: the ring's (phi - phi0)/dphi
+        // plus 1/2 on rings whose first pixel is shifted by half a slot
+        const double half = R.phi0(r) > 0.0 ? 0.5 : 0.0;
+        const double base = R.base(r) + half;
+        const double tc = fma(dl, R.inv_dphi(r), base);
+        const int j = (int)tc;
+        const double f = tc - (double)j;
+        ok &= (tc >= 0.0) & (j + 1 < R.width(r)) & (f > kCertSlot) & (f < 1.0 - kCertSlot);
+        const double ss = sth * R.sth(r), zz = z * R.z(r), dphi = R.dphi(r);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            // phi_w - phi_c = dl + dphi (base - half - (j + q)) in window-relative slots
+            const double g = fma(dphi, base - half - (double)(j + q), dl);
+            ok &= fabs(g) <= 0.05;
+            const double d = fma(ss, cos_tiny(g), zz);
+            const bool win = d > best;
+            second = win ? best : fmax(second, d);
+            cell = win ? R.off(r) + j + q : cell;
+            best = win ? d : best;
+        }
+    }
+    ok &= best - second > kCertNearest;
+    if (!ok) return false;
+    atomicAdd(W + cell, bv);
+    return true;
+}
+
 // Waves 1..7: phase 1 of one sample -- sweep the beam and scatter
 // beam_val * weight into the LDS weight grid; returns the outside-grid sums.
 // (Wave 0 meanwhile lays out the next sample.)
@@ -1084,6 +1179,7 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, cons
             }
             bool ok = false;
             if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
+            if constexpr (MODE == 1) ok = fp_fast_nearest(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
             if (!ok) slow |= 1ull << it;
         }
         // exact path for the beam pixels not certified above
@@ -1118,8 +1214,6 @@ struct CorrRegs {
     bf16x8 w1[4];
     bf16x8 w2[4][2];
     bf16x8 w3[2];
-    f32x4 b2[4];
-    f32x4 b3;
 };
 
 __device__ __forceinline__ int corr_perm(int kb, int g, int j) {
@@ -1142,15 +1236,15 @@ __device__ __forceinline__ void corr_load(const float* __restrict__ P, CorrRegs&
         for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
             for (int j = 0; j < 8; ++j) c.w2[ft][kb][j] = (__bf16)P[kOffW2 + o * kH + corr_perm(kb, g, j)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) c.b2[ft][q] = P[kOffB2 + 16 * ft + 4 * g + q];
     }
 #pragma unroll
     for (int kb = 0; kb < 2; ++kb)
 #pragma unroll
         for (int j = 0; j < 8; ++j) c.w3[kb][j] = (__bf16)P[kOffW3 + r * kH + corr_perm(kb, g, j)];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) c.b3[q] = P[kOffB3 + 4 * g + q];
+    // b2, b3 (FP32) go to LDS: each lane reads the four of its output rows
+    float* bias = &lds_at<float>(kOffBias);
+    for (int i = threadIdx.x; i < kH + kOut; i += kFpThreads)
+        bias[i] = i < kH ? P[kOffB2 + i] : P[kOffB3 + i - kH];
 }
 
 __device__ __forceinline__ bf16x8 corr_act(const f32x4& lo, const f32x4& hi) {
@@ -1168,6 +1262,7 @@ __device__ __forceinline__ bf16x8 corr_act(const f32x4& lo, const f32x4& hi) {
 // on return o[] holds the lane's 16 outputs (the tile is reused for them).
 __device__ __forceinline__ void corr_mlp(const CorrRegs& c, unsigned char* tile, float o[16]) {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const f32x4* bias = reinterpret_cast<const f32x4*>(&lds_at<float>(kOffBias));  // [ft * 4 + g], b3 at [16 + g]
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         unsigned char* row = tile + (16 * t + r) * 64 + 16 * g;
@@ -1180,12 +1275,12 @@ __device__ __forceinline__ void corr_mlp(const CorrRegs& c, unsigned char* tile,
         const bf16x8 a1 = corr_act(h[2], h[3]);
 #pragma unroll
         for (int ft = 0; ft < 4; ++ft) {
-            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][0], a0, c.b2[ft], 0, 0, 0);
+            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][0], a0, bias[ft * 4 + g], 0, 0, 0);
             h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][1], a1, h[ft], 0, 0, 0);
         }
         const bf16x8 e0 = corr_act(h[0], h[1]);
         const bf16x8 e1 = corr_act(h[2], h[3]);
-        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[0], e0, c.b3, 0, 0, 0);
+        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[0], e0, bias[16 + g], 0, 0, 0);
         out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[1], e1, out, 0, 0, 0);
         *reinterpret_cast<f32x4*>(row) = out;  // row 16t + r, outputs 4g .. 4g + 3
     }
@@ -1441,25 +1536,28 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
     return acc;
 }
 
-// Waves 1..7: phase 1 of one sample with the corrector.  Each wave takes 64
-// beam pixels at a time (one per lane): geometry and the 16 neighbourhood
-// values (fast path from the value grid, else the exact path), the MLP on
-// MFMA over the wave's 64 rows, then the corrected weights times the beam
-// value into the weight grid.  Where the corrector does not apply (the
-// rings next to the poles) or its correction exceeds the bound, the plain
-// bilinear weights are used.
+// Phase 1 of one sample with the corrector, by every wave: wave 0 joins
+// when it has laid out the next sample.  Waves claim 64 beam pixels at a
+// time (one per lane) from the sample's LDS counter: geometry and the 16
+// neighbourhood values (fast path from the value grid, else the exact
+// path), the MLP on MFMA over the wave's 64 rows, then the corrected
+// weights times the beam value into the weight grid.  Where the corrector
+// does not apply (the rings next to the poles) or its correction exceeds
+// the bound, the plain bilinear weights are used.
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, const FpJob* J, const FpFrame* fr,
                                                const FpRings& R, int buf, double* W,
                                                const CorrRegs& C, unsigned long long& n_corr,
                                                unsigned long long& n_fb) {
-    const int tid = threadIdx.x - 64;
+    const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nside = a.nside;
-    unsigned char* tile = fp_smem + kOffStage + (wave - 1) * kStageBytes;
+    unsigned char* tile = fp_smem + kOffStage + wave * kStageBytes;
+    int* cnt = &lds_at<int>(kOffCnt + 4 * buf);
     const float* __restrict__ vec = uniform_ptr(J->vec_orig);
     const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
     const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
+    const int64_t n_chunks = (S + 63) >> 6;
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     const float bound = a.corr_bound;
@@ -1468,8 +1566,12 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, const FpJob* J,
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
     Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};
-    for (int64_t s0 = 0; s0 < S; s0 += kP1Threads) {
-        const int64_t s = s0 + tid;
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(cnt, 1);
+        const int64_t chunk = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        if (chunk >= n_chunks) break;
+        const int64_t s = (chunk << 6) + lane;
         const bool live = s < S;
         float u0 = 0.f, u1 = 0.f, u2 = 0.f, bvf = 0.f;
         if (live) {
@@ -1691,12 +1793,15 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
             fp_layout(an, bn, bfn, fp_trig(), fp_frame(buf ^ 1), fp_rings(buf ^ 1));
         }
     } else {
-        if constexpr (MODE == 3)
-            acc = fp_phase1_corr<QU, CS>(an, fp_job(buf), fr, R, buf, fp_W(), *C, *n_corr, *n_fb);
-        else
-            acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
+        if constexpr (MODE != 3) acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
     }
+    if constexpr (MODE == 3)
+        acc = fp_phase1_corr<QU, CS>(an, fp_job(buf), fr, R, buf, fp_W(), *C, *n_corr, *n_fb);
     __syncthreads();
+    // mode 3: every wave has seen this sample's counter run out; it is
+    // reused two samples later
+    if constexpr (MODE == 3)
+        if (threadIdx.x == 0) lds_at<int>(kOffCnt + 4 * buf) = 0;
     acc = fp_phase2<QU, CS>(an, fr, R, fp_W(), acc);
     // mode 3: the next sample's values go into the value grid, which no
     // wave reads between the barrier above and the one below
@@ -1744,6 +1849,10 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     if constexpr (MODE == 3) {
         corr_load(a.corr, C);
         fp_load_values<CS>(a, fp_frame(0), fp_rings(0));
+        if (threadIdx.x == 0) {
+            lds_at<int>(kOffCnt) = 0;
+            lds_at<int>(kOffCnt + 4) = 0;
+        }
         __syncthreads();
     }
     int buf = 0;
