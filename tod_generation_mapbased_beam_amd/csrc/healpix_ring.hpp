@@ -255,6 +255,57 @@ __device__ __forceinline__ void spin2(double z_pix, double sth_pix, double phi_p
     spin2_from_half(spin2_ring(z_pix, sth_pix, z_pts, sth_pts), sh, ch, c2d, s2d);
 }
 
+// Binary32 spin-2 for the footprint sweep: the same formula from the
+// per-ring constants rounded to binary32 (each term keeps ~1e-7 relative
+// accuracy: near the boresight all of N and D are sums of like-signed small
+// terms).  cos 2delta and sin 2delta come out with absolute error ~1e-7,
+// well inside the tolerance of the tod they rotate.
+struct Spin2RingF {
+    float st2, S, s2dt, zsum, SC4;
+};
+
+__device__ __forceinline__ void spin2_from_half_f(const Spin2RingF& r, float sh, float ch, float& c2d,
+                                                  float& s2d) {
+    const float sin_dphi = 2.f * ch * sh;
+    const float sh2 = sh * sh;
+    const float cos_dphi = fmaf(-2.f, sh2, 1.f);
+    const float h = fmaf(r.S, sh2, r.st2);
+    float N = 2.f * sin_dphi * r.zsum * h;
+    float D = fmaf(r.S * sin_dphi, sin_dphi, fmaf(r.s2dt, cos_dphi, -(r.SC4 * sh2 * sh2)));
+    const float mx = fmaxf(fabsf(N), fabsf(D));
+    if (mx == 0.f) {
+        c2d = 1.f;
+        s2d = 0.f;
+        return;
+    }
+    const float im = __builtin_amdgcn_rcpf(mx);  // scale to [1, 2] before squaring
+    N *= im;
+    D *= im;
+    const float u = __builtin_amdgcn_rcpf(fmaf(N, N, D * D));
+    c2d = fmaf(D, D, -(N * N)) * u;
+    s2d = 2.f * N * D * u;
+}
+
+// sin and cos in binary32: Taylor for |x| <= 0.05 (truncation < 3e-15),
+// the library's sincosf beyond (out of line).
+__device__ __noinline__ float2 sincosf_wide(float x) {
+    float s, c;
+    sincosf(x, &s, &c);
+    return make_float2(s, c);
+}
+
+__device__ __forceinline__ void sincos_any_f(float x, float& s, float& c) {
+    if (fabsf(x) <= 0.05f) {
+        const float x2 = x * x;
+        s = fmaf(x * x2, fmaf(x2, 1.f / 120.f, -1.f / 6.f), x);
+        c = fmaf(x2, fmaf(x2, fmaf(x2, -1.f / 720.f, 1.f / 24.f), -0.5f), 1.f);
+    } else {
+        const float2 r = sincosf_wide(x);
+        s = r.x;
+        c = r.y;
+    }
+}
+
 // theta - theta_a for a query z below ring a (z <= za): asin(sin(theta -
 // theta_a)) with sin(theta - theta_a) = sqrt(1-z^2) za - z sqrt(1-za^2),
 // by its series when small, else via acos.  Weights only.

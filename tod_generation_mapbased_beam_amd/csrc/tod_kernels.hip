@@ -461,13 +461,11 @@ constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
 constexpr int kOffW = kOffJob + 2 * 64;
 constexpr int kFpSmemBytes = kOffW + kFpCap * 8;
 // mode 3 (learned corrector) adds the driving field's values of the
-// footprint cells, one 64 x 64-byte staging tile per wave and the two
-// per-buffer chunk counters of phase 1
+// footprint cells, one 64 x 64-byte staging tile per wave and the biases
 constexpr int kOffIv = kFpSmemBytes;
 constexpr int kOffStage = kOffIv + kFpCap * 4;
 constexpr int kStageBytes = 64 * 64;
-constexpr int kOffCnt = kOffStage + kFpWaves * kStageBytes;
-constexpr int kOffBias = kOffCnt + 16;                   // b2 (64), b3 (16) as float
+constexpr int kOffBias = kOffStage + kFpWaves * kStageBytes;  // b2 (64), b3 (16) as float
 constexpr int kFpSmemBytesCorr = kOffBias + 80 * 4;
 static_assert(kOffStage % 16 == 0 && kOffBias % 16 == 0, "16-byte LDS vectors");
 static_assert(kFpSmemBytesCorr <= 160 * 1024, "one workgroup per CU");
@@ -489,12 +487,15 @@ struct FpRings {
     __device__ double& inv_dth(int r) const { return d(4, r); }
     __device__ double& phi0(int r) const { return d(5, r); }
     __device__ double& dphi(int r) const { return d(6, r); }
-    // spin-2 constants of the ring against the sample's boresight (Spin2Ring)
-    __device__ double& st2(int r) const { return d(7, r); }
-    __device__ double& sS(int r) const { return d(8, r); }
-    __device__ double& s2dt(int r) const { return d(9, r); }
-    __device__ double& zsum(int r) const { return d(10, r); }
-    __device__ double& sc4(int r) const { return d(11, r); }
+    // spin-2 constants of the ring against the sample's boresight
+    // (Spin2Ring), rounded to binary32 for the footprint sweep; each sits in
+    // the first half of an 8-byte slot
+    __device__ float& f(int k, int r) const { return lds_at<float>(o + 8 * (k * kFpMaxRings + r)); }
+    __device__ float& st2(int r) const { return f(7, r); }
+    __device__ float& sS(int r) const { return f(8, r); }
+    __device__ float& s2dt(int r) const { return f(9, r); }
+    __device__ float& zsum(int r) const { return f(10, r); }
+    __device__ float& sc4(int r) const { return f(11, r); }
     __device__ int& width(int r) const { return i(0, r); }
     __device__ int& off(int r) const { return i(1, r); }    // non-decreasing in r
     __device__ int& i0(int r) const { return i(2, r); }
@@ -512,12 +513,14 @@ __device__ __forceinline__ double* fp_scratch() { return &lds_at<double>(kOffScr
 __device__ __forceinline__ double* fp_W() { return &lds_at<double>(kOffW); }
 
 // The per-sample operands a stage needs after the sample's layout: the
-// beam, the tod destination and the sample index (one slot per buffer).
+// beam, the tod destination, the sample index and phase 1's claim counter
+// (one slot per buffer).
 struct alignas(16) FpJob {
     const float* vec_orig;
     const float* beam_vals;
     double* tod;
     int64_t S, B, b;
+    int claim;  // next chunk of beam pixels (LDS atomic counter)
 };
 static_assert(sizeof(FpJob) <= 64, "job slot");
 __device__ __forceinline__ FpJob* fp_job(int buf) { return &lds_at<FpJob>(kOffJob + 64 * buf); }
@@ -863,11 +866,11 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
                 base = tc >= (double)g.n ? tc - (double)g.n : (tc < 0.0 ? tc + (double)g.n : tc);
             }
             const Spin2Ring sr = spin2_ring(g.z, g.sth, z_pts, sth_pts);
-            R.st2(r) = sr.st2;
-            R.sS(r) = sr.S;
-            R.s2dt(r) = sr.s2dt;
-            R.zsum(r) = sr.zsum;
-            R.sc4(r) = sr.SC4;
+            R.st2(r) = (float)sr.st2;
+            R.sS(r) = (float)sr.S;
+            R.s2dt(r) = (float)sr.s2dt;
+            R.zsum(r) = (float)sr.zsum;
+            R.sc4(r) = (float)sr.SC4;
             R.base(r) = base;
             R.inv_dphi(r) = g.inv_dphi;
             R.z(r) = g.z;
@@ -1138,15 +1141,16 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
     return true;
 }
 
-// Waves 1..7: phase 1 of one sample -- sweep the beam and scatter
-// beam_val * weight into the LDS weight grid; returns the outside-grid sums.
-// (Wave 0 meanwhile lays out the next sample.)
-constexpr int kP1Threads = kFpThreads - 64;
+// Phase 1 of one sample -- sweep the beam and scatter beam_val * weight
+// into the LDS weight grid; returns the outside-grid sums.  Waves claim
+// kClaim beam pixels at a time from the sample's counter; wave 0 joins once
+// it has laid out the next sample.
+constexpr int kClaim = 256;  // beam pixels per claim (4 per lane)
 
 template <int MODE, bool QU, int CS>
-__device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, const FpFrame* fr,
+__device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFrame* fr,
                                           const FpRings& R, int buf, double* W) {
-    const int tid = threadIdx.x - 64;
+    const int lane = threadIdx.x & 63;
     const int nside = a.nside;
     const float* __restrict__ vec = uniform_ptr(J->vec_orig);
     const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
@@ -1157,21 +1161,27 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, cons
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
     Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};  // outside-grid sums
-    for (int64_t s0 = 0; s0 < S; s0 += (int64_t)kP1Threads * 64) {
-        uint64_t slow = 0;
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = atomicAdd(&J->claim, 1);
+        const int64_t s0 = (int64_t)__builtin_amdgcn_readfirstlane(__shfl(c, 0, 64)) * kClaim;
+        if (s0 >= S) break;
+        uint32_t slow = 0;
+        int64_t s = s0 + lane;
         float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
-        int64_t s = s0 + tid;
         if (s < S) {
             nv0 = vec[3 * s];
             nv1 = vec[3 * s + 1];
             nv2 = vec[3 * s + 2];
             nbv = bvs[s];
         }
-        for (int it = 0; it < 64 && s < S; ++it, s += kP1Threads) {
+#pragma unroll
+        for (int it = 0; it < kClaim / 64; ++it, s += 64) {
+            if (s >= S) break;
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
             const double bv = (double)nbv;
-            const int64_t sn = s + kP1Threads;
-            if (it + 1 < 64 && sn < S) {  // fetch the next beam pixel early
+            const int64_t sn = s + 64;
+            if (it + 1 < kClaim / 64 && sn < S) {  // fetch the next beam pixel early
                 nv0 = vec[3 * sn];
                 nv1 = vec[3 * sn + 1];
                 nv2 = vec[3 * sn + 2];
@@ -1180,13 +1190,13 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, const FpJob* J, cons
             bool ok = false;
             if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
             if constexpr (MODE == 1) ok = fp_fast_nearest(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
-            if (!ok) slow |= 1ull << it;
+            if (!ok) slow |= 1u << it;
         }
         // exact path for the beam pixels not certified above
         while (slow) {
-            const int it = __builtin_ctzll(slow);
+            const int it = __builtin_ctz(slow);
             slow &= slow - 1;
-            const int64_t ss = s0 + (int64_t)it * kP1Threads + tid;
+            const int64_t ss = s0 + (int64_t)it * 64 + lane;
             const Acc4 d = fp_exact_pixel<MODE, QU, CS>(fp_globals(a), buf, vec[3 * ss],
                                                         vec[3 * ss + 1], vec[3 * ss + 2], bvs[ss]);
 #pragma unroll
@@ -1545,7 +1555,7 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
 // does not apply (the rings next to the poles) or its correction exceeds
 // the bound, the plain bilinear weights are used.
 template <bool QU, int CS>
-__device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, const FpJob* J, const FpFrame* fr,
+__device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const FpFrame* fr,
                                                const FpRings& R, int buf, double* W,
                                                const CorrRegs& C, unsigned long long& n_corr,
                                                unsigned long long& n_fb) {
@@ -1553,7 +1563,7 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, const FpJob* J,
     const int wave = threadIdx.x >> 6;
     const int nside = a.nside;
     unsigned char* tile = fp_smem + kOffStage + wave * kStageBytes;
-    int* cnt = &lds_at<int>(kOffCnt + 4 * buf);
+    int* cnt = &J->claim;
     const float* __restrict__ vec = uniform_ptr(J->vec_orig);
     const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
     const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
@@ -1671,6 +1681,12 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, const FpJob* J,
     return accd;
 }
 
+// Footprint pixels whose ring lies within sin^2(d/2) < kSpinF32St2 of the
+// boresight (d < ~2 deg) and whose half longitude gap is small take the
+// binary32 spin-2 (absolute error ~1e-7); the rest, where its terms can
+// cancel, the binary64 one.
+constexpr float kSpinF32St2 = 3e-4f;
+
 // All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
 // gather per map pixel with nonzero weight; the cell is cleared for the next
 // sample.  Each wave takes 64 consecutive grid cells at a time (one per
@@ -1702,21 +1718,32 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
         const float4 mv = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
         if constexpr (QU) {
             double c2d, s2d;
+            // the half longitude gap relative to lam (a difference of angles)
+            const double hd = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
+            const float st2 = R.st2(r);
             if (ir_lo + r == sp.coinc_ring && pos == sp.coinc_pos) {  // tod_bilinear.py:108-109
                 c2d = 1.0;
                 s2d = 0.0;
-            } else {
-                const double hb = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
-                double sb, cb;
-                sincos_any(hb, sb, cb);
-                const double sh = fma(sp.sha, cb, -(sp.cha * sb));
-                const double ch = fma(sp.cha, cb, sp.sha * sb);
-                Spin2Ring g;
-                g.st2 = R.st2(r);
+            } else if (st2 < kSpinF32St2 && fabs(hd) < 0.05) {
+                // pixel within ~2 deg of the boresight: binary32
+                float sb, cb, cf, sf;
+                sincos_any_f((float)hd, sb, cb);
+                const float sha = (float)sp.sha, cha = (float)sp.cha;
+                Spin2RingF g;
+                g.st2 = st2;
                 g.S = R.sS(r);
                 g.s2dt = R.s2dt(r);
                 g.zsum = R.zsum(r);
                 g.SC4 = R.sc4(r);
+                spin2_from_half_f(g, fmaf(sha, cb, -(cha * sb)), fmaf(cha, cb, sha * sb), cf, sf);
+                c2d = (double)cf;
+                s2d = (double)sf;
+            } else {
+                double sb, cb;
+                sincos_any(hd, sb, cb);
+                const double sh = fma(sp.sha, cb, -(sp.cha * sb));
+                const double ch = fma(sp.cha, cb, sp.sha * sb);
+                const Spin2Ring g = spin2_ring(R.z(r), R.sth(r), sp.z_pts, sp.sth_pts);
                 spin2_from_half(g, sh, ch, c2d, s2d);
             }
             const double qv = (double)mv.x, uv = (double)mv.y;
@@ -1768,6 +1795,7 @@ __device__ __forceinline__ void fp_set_job(const TodArgs& a, int64_t b, int buf)
         J->S = a.S;
         J->B = a.B;
         J->b = b;
+        J->claim = 0;
     }
 }
 
@@ -1787,21 +1815,17 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     const FpRings R = fp_rings(buf);
     const FpFrame* fr = fp_frame(buf);
     Acc4 acc = {{0.0, 0.0, 0.0, 0.0}};
-    if (wave == 0) {
-        if (have_next) {
-            fp_set_job(an, bn, buf ^ 1);
-            fp_layout(an, bn, bfn, fp_trig(), fp_frame(buf ^ 1), fp_rings(buf ^ 1));
-        }
-    } else {
-        if constexpr (MODE != 3) acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
+    if (wave == 0 && have_next) {
+        fp_set_job(an, bn, buf ^ 1);
+        fp_layout(an, bn, bfn, fp_trig(), fp_frame(buf ^ 1), fp_rings(buf ^ 1));
     }
+    // every wave, wave 0 once its layout is done: the job's claim counter
+    // was zeroed when the job was set
     if constexpr (MODE == 3)
         acc = fp_phase1_corr<QU, CS>(an, fp_job(buf), fr, R, buf, fp_W(), *C, *n_corr, *n_fb);
+    else
+        acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
     __syncthreads();
-    // mode 3: every wave has seen this sample's counter run out; it is
-    // reused two samples later
-    if constexpr (MODE == 3)
-        if (threadIdx.x == 0) lds_at<int>(kOffCnt + 4 * buf) = 0;
     acc = fp_phase2<QU, CS>(an, fr, R, fp_W(), acc);
     // mode 3: the next sample's values go into the value grid, which no
     // wave reads between the barrier above and the one below
@@ -1849,10 +1873,6 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     if constexpr (MODE == 3) {
         corr_load(a.corr, C);
         fp_load_values<CS>(a, fp_frame(0), fp_rings(0));
-        if (threadIdx.x == 0) {
-            lds_at<int>(kOffCnt) = 0;
-            lds_at<int>(kOffCnt + 4) = 0;
-        }
         __syncthreads();
     }
     int buf = 0;
