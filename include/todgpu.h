@@ -64,7 +64,7 @@ int todg_pack_map(const float *mp_stacked, int C, int64_t npix, int c_q, int c_u
  * 0 = footprint kernel (per-sample LDS weight grid, 4 waves/SIMD; the
  * default), 8 = the footprint kernel compiled for 2 waves/SIMD, 1/2/4 =
  * direct kernel with that many samples per wavefront.  Results agree to
- * rounding; neighbour indices are identical. */
+ * rounding; neighbour indices are identical.  S <= 2^28 (status -1 beyond). */
 int todg_gather_accum_fused(const float *vec_orig, const float *axes, const float *cos_a,
                             const float *sin_a, const float *ax_pts, const float *cos_p,
                             const float *sin_p, int64_t nside, const float *map_packed, int C,
@@ -112,7 +112,8 @@ typedef struct todg_work_item {
 /* Many detectors with their own beams (uneven S) and scan chunks (uneven B)
  * in one persistent launch (the reference calls beam_tod_batch once per
  * beam entry per batch: sample_based_tod_generation_gridint.py:284-296).
- * mode 0 = bilinear, 1 = nearest, 2 = bicubic; items: device array of n_items; the map,
+ * mode 0 = bilinear, 1 = nearest, 2 = bicubic; items: device array of n_items (S <= 2^28
+ * each); the map,
  * C and c_q/c_u are shared.  chunk = samples per work-queue claim (1..65536);
  * at most 2^33 samples per launch.
  * queue: device scratch of >= 8 unsigned ints (zeroed by the call itself,

@@ -60,6 +60,8 @@ class BatchPlan:
             ps = [to_dev(p, f32) for p in d.params]
             B = int(ps[1].numel())
             S = int(bv.numel())
+            if S > (1 << 28):
+                raise ValueError(f"detector {i}: at most 2^28 beam pixels")
             tod = d.tod if d.tod is not None else torch.zeros((dmap.C, B), dtype=torch.float64, device=dev)
             if tod.dtype != torch.float64 or tuple(tod.shape) != (dmap.C, B) or not tod.is_contiguous():
                 raise ValueError(f"detector {i}: tod must be contiguous float64 ({dmap.C}, {B})")

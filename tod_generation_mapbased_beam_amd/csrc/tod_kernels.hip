@@ -404,8 +404,18 @@ constexpr int kFpWaves = kFpThreads / 64;
 constexpr double kCertSlot = 1e-9;
 constexpr double kCertPolar = 1e-6;
 // Largest |tan| of the longitude offset from the reference longitude that
-// the fast path's atan series takes (truncation t^15/15 < 1e-19 at 0.06).
-constexpr double kMaxTan = 0.06;
+// the fast path's atan series takes (five terms: truncation t^11/11 < 2e-18
+// at 0.03; the offsets of beams up to ~1 deg across stay below it).
+constexpr double kMaxTan = 0.03;
+
+// atan(t) for |t| < kMaxTan.
+__device__ __forceinline__ double atan_small(double t) {
+    const double t2 = t * t;
+    double p = fma(t2, 1.0 / 9.0, -1.0 / 7.0);
+    p = fma(t2, p, 1.0 / 5.0);
+    p = fma(t2, p, -1.0 / 3.0);
+    return fma(t * t2, p, t);
+}
 
 // Per-sample values shared by the workgroup (written by wave 0).
 struct alignas(16) FpFrame {
@@ -528,6 +538,18 @@ __device__ __forceinline__ FpJob* fp_job(int buf) { return &lds_at<FpJob>(kOffJo
 template <typename T>
 __device__ __forceinline__ T* uniform_ptr(T* p) {
     return reinterpret_cast<T*>(__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, reinterpret_cast<uint64_t>(p)))));
+}
+
+// A pointer read back from LDS carries no address space, so loads through
+// it compile to FLAT instructions, which also count against the LDS wait
+// counter: every LDS wait would then wait for those global loads too.
+// These casts restore the global address space.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<const T> global_in(const T* p) {
+    return (gptr<const T>)uniform_ptr(p);
 }
 
 // Local index of (footprint ring r, slot pos in [0, n]) in the weight grid, or -1.
@@ -784,18 +806,21 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
     const int lane = threadIdx.x & 63;
     const int nside = a.nside;
     Sample q;
-    q.kx = (double)a.axes[3 * b];
-    q.ky = (double)a.axes[3 * b + 1];
-    q.kz = (double)a.axes[3 * b + 2];
-    q.ca = (double)a.cos_a[b];
-    q.sa = (double)a.sin_a[b];
-    q.oma = 1.0 - q.ca;
-    q.bx = (double)a.ax_pts[3 * b];
-    q.by = (double)a.ax_pts[3 * b + 1];
-    q.bz = (double)a.ax_pts[3 * b + 2];
-    q.cp = (double)a.cos_p[b];
-    q.sp = (double)a.sin_p[b];
-    q.omp = 1.0 - q.cp;
+    {
+        const gptr<const float> axes = global_in(a.axes), ax_pts = global_in(a.ax_pts);
+        q.kx = (double)axes[3 * b];
+        q.ky = (double)axes[3 * b + 1];
+        q.kz = (double)axes[3 * b + 2];
+        q.ca = (double)global_in(a.cos_a)[b];
+        q.sa = (double)global_in(a.sin_a)[b];
+        q.oma = 1.0 - q.ca;
+        q.bx = (double)ax_pts[3 * b];
+        q.by = (double)ax_pts[3 * b + 1];
+        q.bz = (double)ax_pts[3 * b + 2];
+        q.cp = (double)global_in(a.cos_p)[b];
+        q.sp = (double)global_in(a.sin_p)[b];
+        q.omp = 1.0 - q.cp;
+    }
     make_uniform(q);
     double fx, fy, fz;  // footprint centre = this sample's image of c
     rotate(q, bf.cx, bf.cy, bf.cz, fx, fy, fz);
@@ -1016,15 +1041,8 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     ok &= (ira >= 1) & (ira <= 4 * nside - 2) & (la >= 0) & (la + 1 < nr);
     ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
     la = ok ? la : 0;
-    // longitude offset from lam: atan(t), |t| < 0.06
-    const double t = yp * rcp_nr(xp);
-    const double t2 = t * t;
-    double p = fma(t2, 1.0 / 13.0, -1.0 / 11.0);
-    p = fma(t2, p, 1.0 / 9.0);
-    p = fma(t2, p, -1.0 / 7.0);
-    p = fma(t2, p, 1.0 / 5.0);
-    p = fma(t2, p, -1.0 / 3.0);
-    const double dl = fma(t * t2, p, t);
+    // longitude offset from lam: atan(t), |t| < kMaxTan
+    const double dl = atan_small(yp * rcp_nr(xp));
     const double ta = fma(dl, R.inv_dphi(la), R.base(la));
     const double tb = fma(dl, R.inv_dphi(la + 1), R.base(la + 1));
     const int ia = (int)ta, ib = (int)tb;
@@ -1036,18 +1054,23 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     const double x = fma(sq, R.z(la), -(z * R.sth(la)));
     ok &= fabs(x) < 1e-2;
     if (!ok) return false;
-    const double x2 = x * x;
-    double pp = fma(x2, 105.0 / 3456.0, 15.0 / 336.0);
-    pp = fma(x2, pp, 3.0 / 40.0);
-    pp = fma(x2, pp, 1.0 / 6.0);
-    const double wb = fma(x * x2, pp, x) * R.inv_dth(la) * bv;
-    const double wa = bv - wb;
+    // weights in binary32 (relative error ~1e-7): asin(x) by its series,
+    // |x| < 1e-2 (x itself needed binary64: it is a difference of O(1) terms)
+    const float xf = (float)x;
+    const float x2 = xf * xf;
+    float pp = fmaf(x2, 105.f / 3456.f, 15.f / 336.f);
+    pp = fmaf(x2, pp, 3.f / 40.f);
+    pp = fmaf(x2, pp, 1.f / 6.f);
+    const float bvf = (float)bv;
+    const float wb = fmaf(xf * x2, pp, xf) * (float)R.inv_dth(la) * bvf;
+    const float wa = bvf - wb;
+    const float faf = (float)fa, fbf = (float)fb;
     double* ca = W + R.off(la) + ia;
     double* cb = W + R.off(la + 1) + ib;
-    atomicAdd(ca, wa - wa * fa);
-    atomicAdd(ca + 1, wa * fa);
-    atomicAdd(cb, wb - wb * fb);
-    atomicAdd(cb + 1, wb * fb);
+    atomicAdd(ca, (double)(wa - wa * faf));
+    atomicAdd(ca + 1, (double)(wa * faf));
+    atomicAdd(cb, (double)(wb - wb * fbf));
+    atomicAdd(cb + 1, (double)(wb * fbf));
     return true;
 }
 
@@ -1100,14 +1123,7 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
     ok &= (ira >= 1) & (ira <= 4 * nside - 2) & (la >= 0) & (la + 1 < nr);
     ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
     la = ok ? la : 0;
-    const double t = yp * rcp_nr(xp);
-    const double t2 = t * t;
-    double p = fma(t2, 1.0 / 13.0, -1.0 / 11.0);
-    p = fma(t2, p, 1.0 / 9.0);
-    p = fma(t2, p, -1.0 / 7.0);
-    p = fma(t2, p, 1.0 / 5.0);
-    p = fma(t2, p, -1.0 / 3.0);
-    const double dl = fma(t * t2, p, t);
+    const double dl = atan_small(yp * rcp_nr(xp));
     const double sth = sqrt_fast(fmax(fma(xp, xp, yp * yp), 1e-300));
     double best = -2.0, second = -2.0;
     int cell = 0;
@@ -1152,56 +1168,67 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
                                           const FpRings& R, int buf, double* W) {
     const int lane = threadIdx.x & 63;
     const int nside = a.nside;
-    const float* __restrict__ vec = uniform_ptr(J->vec_orig);
-    const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
-    const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
+    const gptr<const float> __restrict__ vec = global_in(J->vec_orig);
+    const gptr<const float> __restrict__ bvs = global_in(J->beam_vals);
+    // S <= 2^28 (checked on the host; clamped here so that 3 S stays in int range)
+    const int64_t S64 = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
+    const int S = (int)(S64 < (int64_t(1) << 28) ? S64 : (int64_t(1) << 28));
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     double m[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
     Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};  // outside-grid sums
-    for (;;) {
+    // claims are pipelined: the next chunk is claimed, and its first beam
+    // pixel fetched, while the current chunk is swept
+    auto claim = [&]() {
         int c = 0;
         if (lane == 0) c = atomicAdd(&J->claim, 1);
-        const int64_t s0 = (int64_t)__builtin_amdgcn_readfirstlane(__shfl(c, 0, 64)) * kClaim;
-        if (s0 >= S) break;
+        return __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64)) * kClaim;
+    };
+    int s0 = claim();
+    float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
+    if (s0 + lane < S) {
+        nv0 = vec[3 * (s0 + lane)];
+        nv1 = vec[3 * (s0 + lane) + 1];
+        nv2 = vec[3 * (s0 + lane) + 2];
+        nbv = bvs[s0 + lane];
+    }
+    while (s0 < S) {
+        const int s1 = claim();
         uint32_t slow = 0;
-        int64_t s = s0 + lane;
-        float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
-        if (s < S) {
-            nv0 = vec[3 * s];
-            nv1 = vec[3 * s + 1];
-            nv2 = vec[3 * s + 2];
-            nbv = bvs[s];
-        }
 #pragma unroll
-        for (int it = 0; it < kClaim / 64; ++it, s += 64) {
-            if (s >= S) break;
+        for (int it = 0; it < kClaim / 64; ++it) {
+            const int s = s0 + it * 64 + lane;
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
             const double bv = (double)nbv;
-            const int64_t sn = s + 64;
-            if (it + 1 < kClaim / 64 && sn < S) {  // fetch the next beam pixel early
+            // fetch the next beam pixel early (the next chunk's first one
+            // after the last iteration)
+            const int sn = it + 1 < kClaim / 64 ? s + 64 : s1 + lane;
+            if (sn < S) {
                 nv0 = vec[3 * sn];
                 nv1 = vec[3 * sn + 1];
                 nv2 = vec[3 * sn + 2];
                 nbv = bvs[sn];
             }
-            bool ok = false;
-            if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
-            if constexpr (MODE == 1) ok = fp_fast_nearest(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
-            if (!ok) slow |= 1u << it;
+            if (s < S) {
+                bool ok = false;
+                if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
+                if constexpr (MODE == 1) ok = fp_fast_nearest(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
+                if (!ok) slow |= 1u << it;
+            }
         }
         // exact path for the beam pixels not certified above
         while (slow) {
             const int it = __builtin_ctz(slow);
             slow &= slow - 1;
-            const int64_t ss = s0 + (int64_t)it * 64 + lane;
+            const int ss = s0 + it * 64 + lane;
             const Acc4 d = fp_exact_pixel<MODE, QU, CS>(fp_globals(a), buf, vec[3 * ss],
                                                         vec[3 * ss + 1], vec[3 * ss + 2], bvs[ss]);
 #pragma unroll
             for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
         }
+        s0 = s1;
     }
     return accd;
 }
@@ -1380,14 +1407,7 @@ __device__ __forceinline__ bool fp_fast_corr(const double* __restrict__ m, int n
     ok &= (ira >= 2) & (ira <= 4 * nside - 3) & (la >= 1) & (la + 2 < nr);
     ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
     la = ok ? la : 1;
-    const double t = yp * rcp_nr(xp);
-    const double t2 = t * t;
-    double p = fma(t2, 1.0 / 13.0, -1.0 / 11.0);
-    p = fma(t2, p, 1.0 / 9.0);
-    p = fma(t2, p, -1.0 / 7.0);
-    p = fma(t2, p, 1.0 / 5.0);
-    p = fma(t2, p, -1.0 / 3.0);
-    const double dl = fma(t * t2, p, t);
+    const double dl = atan_small(yp * rcp_nr(xp));
     double fr4[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1564,8 +1584,8 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
     const int nside = a.nside;
     unsigned char* tile = fp_smem + kOffStage + wave * kStageBytes;
     int* cnt = &J->claim;
-    const float* __restrict__ vec = uniform_ptr(J->vec_orig);
-    const float* __restrict__ bvs = uniform_ptr(J->beam_vals);
+    const gptr<const float> __restrict__ vec = global_in(J->vec_orig);
+    const gptr<const float> __restrict__ bvs = global_in(J->beam_vals);
     const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
     const int64_t n_chunks = (S + 63) >> 6;
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
@@ -1768,12 +1788,12 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
 // result deterministic; they need no reply, so nothing waits on them.
 __device__ __forceinline__ void fp_flush(const TodArgs& a, const FpJob* J) {
     if (threadIdx.x == 0) {
-        double* tod = J->tod;
+        const gptr<double> tod = (gptr<double>)J->tod;
         const int64_t B = J->B, b = J->b;
         for (int k = 0; k < kMaxSlots && k < a.nslots; ++k) {
             double sum = 0.0;
             for (int w = 0; w < kFpWaves; ++w) sum += fp_red()[w * kMaxSlots + k];
-            atomicAdd(&tod[(int64_t)a.row[k] * B + b], sum);
+            __builtin_amdgcn_global_atomic_fadd_f64(tod + (int64_t)a.row[k] * B + b, sum);
         }
     }
 }
@@ -2287,6 +2307,7 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
     int nslots = 0;
     int rc = slot_layout(C, c_q, c_u, qu, perm, a->row, &nslots);
     if (rc) return rc;
+    if (S > (int64_t(1) << 28)) return fail(TODG_ERR_ARG, "S must be <= 2^28 beam pixels");
     if (B > 0 && S > 0 &&
         (!vec_orig || !axes || !cos_a || !sin_a || !ax_pts || !cos_p || !sin_p ||
          (need_map && (!map_packed || !beam_vals || !tod))))
