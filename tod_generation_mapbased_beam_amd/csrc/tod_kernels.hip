@@ -1188,23 +1188,28 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
     };
     int s0 = claim();
     float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
-    if (s0 + lane < S) {
-        nv0 = vec[3 * (s0 + lane)];
-        nv1 = vec[3 * (s0 + lane) + 1];
-        nv2 = vec[3 * (s0 + lane) + 2];
-        nbv = bvs[s0 + lane];
+    // lane l sweeps beam pixels s0 + kPer l .. + kPer - 1 of a claim: in one
+    // iteration the wave's lanes sit kPer pixels apart, so fewer of them
+    // scatter into the same weight-grid cell (same-address LDS atomics
+    // serialise)
+    constexpr int kPer = kClaim / 64;
+    if (s0 + lane * kPer < S) {
+        nv0 = vec[3 * (s0 + lane * kPer)];
+        nv1 = vec[3 * (s0 + lane * kPer) + 1];
+        nv2 = vec[3 * (s0 + lane * kPer) + 2];
+        nbv = bvs[s0 + lane * kPer];
     }
     while (s0 < S) {
         const int s1 = claim();
         uint32_t slow = 0;
 #pragma unroll
-        for (int it = 0; it < kClaim / 64; ++it) {
-            const int s = s0 + it * 64 + lane;
+        for (int it = 0; it < kPer; ++it) {
+            const int s = s0 + lane * kPer + it;
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
             const double bv = (double)nbv;
             // fetch the next beam pixel early (the next chunk's first one
             // after the last iteration)
-            const int sn = it + 1 < kClaim / 64 ? s + 64 : s1 + lane;
+            const int sn = it + 1 < kPer ? s + 1 : s1 + lane * kPer;
             if (sn < S) {
                 nv0 = vec[3 * sn];
                 nv1 = vec[3 * sn + 1];
@@ -1222,7 +1227,7 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
         while (slow) {
             const int it = __builtin_ctz(slow);
             slow &= slow - 1;
-            const int ss = s0 + it * 64 + lane;
+            const int ss = s0 + lane * kPer + it;
             const Acc4 d = fp_exact_pixel<MODE, QU, CS>(fp_globals(a), buf, vec[3 * ss],
                                                         vec[3 * ss + 1], vec[3 * ss + 2], bvs[ss]);
 #pragma unroll
@@ -1587,7 +1592,6 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
     const gptr<const float> __restrict__ vec = global_in(J->vec_orig);
     const gptr<const float> __restrict__ bvs = global_in(J->beam_vals);
     const int64_t S = (int64_t)__builtin_bit_cast(uint64_t, uniform(__builtin_bit_cast(double, J->S)));
-    const int64_t n_chunks = (S + 63) >> 6;
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     const float bound = a.corr_bound;
@@ -1596,12 +1600,20 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = uniform(fr->m[k]);
     Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};
+    constexpr int kPer = kClaim / 64;
+    int it = kPer;
+    int64_t s0 = 0;
     for (;;) {
-        int c = 0;
-        if (lane == 0) c = atomicAdd(cnt, 1);
-        const int64_t chunk = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
-        if (chunk >= n_chunks) break;
-        const int64_t s = (chunk << 6) + lane;
+        if (it == kPer) {  // claim the next kClaim beam pixels
+            int c = 0;
+            if (lane == 0) c = atomicAdd(cnt, 1);
+            s0 = (int64_t)__builtin_amdgcn_readfirstlane(__shfl(c, 0, 64)) * kClaim;
+            it = 0;
+        }
+        if (s0 + it >= S) break;  // no lane has a beam pixel left (uniform)
+        // lanes kPer beam pixels apart (fewer same-cell LDS atomics)
+        const int64_t s = s0 + (int64_t)lane * kPer + it;
+        ++it;
         const bool live = s < S;
         float u0 = 0.f, u1 = 0.f, u2 = 0.f, bvf = 0.f;
         if (live) {
@@ -1713,6 +1725,27 @@ constexpr float kSpinF32St2 = 3e-4f;
 // lane), which span one or a few rings; the spin-2 terms that depend only on
 // the ring were computed with the layout.  Returns the per-thread sums
 // (outside-grid sums included).
+__device__ __forceinline__ int fp_ring_of(const FpRings& R, int nr, int r0, int j) {
+    // the ring of cell j >= off(r0): ring offsets are non-decreasing, so
+    // count the next few boundaries at or below j (independent LDS reads,
+    // uniform addresses); a chunk of 64 cells rarely spans more rings
+    int r = r0;
+#pragma unroll
+    for (int k = 1; k <= 4; ++k) r += (r0 + k < nr && R.off(r0 + k) <= j) ? 1 : 0;
+    if (r == r0 + 4)
+        while (r + 1 < nr && R.off(r + 1) <= j) ++r;
+    return r;
+}
+
+// All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
+// gather per map pixel with nonzero weight; the cell is cleared for the next
+// sample.  A wave takes 64 consecutive grid cells at a time (one per lane,
+// spanning one or a few rings; the spin-2 terms that depend only on the
+// ring were computed with the layout), kP2Cells such chunks per round, so
+// that each lane has kP2Cells independent weight reads and gathers in
+// flight.  Returns the per-thread sums (outside-grid sums included).
+constexpr int kP2Cells = 4;
+
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
                                           double* W, Acc4 acc) {
@@ -1722,60 +1755,81 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
     const int total = __builtin_amdgcn_readfirstlane(fr->total);
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const FpSpin sp = fp_spin_of(fr);
-    int r0 = 0;  // ring of the chunk's first cell (uniform, non-decreasing)
-    for (int c0 = wave * 64; c0 < total; c0 += kFpThreads) {
-        while (r0 + 1 < nr && R.off(r0 + 1) <= c0) ++r0;
-        const int j = c0 + lane;
-        if (j >= total) continue;
-        const double wsum = W[j];
-        if (wsum == 0.0) continue;
-        W[j] = 0.0;
-        int r = r0;
-        while (r + 1 < nr && R.off(r + 1) <= j) ++r;
-        int pos = R.i0(r) + (j - R.off(r));
-        const int n = R.n(r);
-        if (pos >= n) pos -= n;
-        const float4 mv = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
-        if constexpr (QU) {
-            double c2d, s2d;
-            // the half longitude gap relative to lam (a difference of angles)
-            const double hd = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
-            const float st2 = R.st2(r);
-            if (ir_lo + r == sp.coinc_ring && pos == sp.coinc_pos) {  // tod_bilinear.py:108-109
-                c2d = 1.0;
-                s2d = 0.0;
-            } else if (st2 < kSpinF32St2 && fabs(hd) < 0.05) {
-                // pixel within ~2 deg of the boresight: binary32
-                float sb, cb, cf, sf;
-                sincos_any_f((float)hd, sb, cb);
-                const float sha = (float)sp.sha, cha = (float)sp.cha;
-                Spin2RingF g;
-                g.st2 = st2;
-                g.S = R.sS(r);
-                g.s2dt = R.s2dt(r);
-                g.zsum = R.zsum(r);
-                g.SC4 = R.sc4(r);
-                spin2_from_half_f(g, fmaf(sha, cb, -(cha * sb)), fmaf(cha, cb, sha * sb), cf, sf);
-                c2d = (double)cf;
-                s2d = (double)sf;
-            } else {
-                double sb, cb;
-                sincos_any(hd, sb, cb);
-                const double sh = fma(sp.sha, cb, -(sp.cha * sb));
-                const double ch = fma(sp.cha, cb, sp.sha * sb);
-                const Spin2Ring g = spin2_ring(R.z(r), R.sth(r), sp.z_pts, sp.sth_pts);
-                spin2_from_half(g, sh, ch, c2d, s2d);
+    int r0 = 0;  // ring of the current chunk's first cell (uniform, non-decreasing)
+    for (int base = wave * 64; base < total; base += kFpThreads * kP2Cells) {
+        double wsum[kP2Cells];
+        int rr[kP2Cells], pp[kP2Cells];
+        float4 mv[kP2Cells];
+#pragma unroll
+        for (int u = 0; u < kP2Cells; ++u) {
+            const int c0 = base + u * kFpThreads;
+            const int j = c0 + lane;
+            wsum[u] = j < total ? W[j] : 0.0;
+            if (c0 < total) r0 = fp_ring_of(R, nr, r0, c0);  // uniform
+            rr[u] = r0;
+        }
+#pragma unroll
+        for (int u = 0; u < kP2Cells; ++u) {
+            const int j = base + u * kFpThreads + lane;
+            pp[u] = 0;
+            if (wsum[u] != 0.0) {
+                W[j] = 0.0;
+                const int r = fp_ring_of(R, nr, rr[u], j);
+                int pos = R.i0(r) + (j - R.off(r));
+                const int n = R.n(r);
+                if (pos >= n) pos -= n;
+                rr[u] = r;
+                pp[u] = pos;
+                mv[u] = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
             }
-            const double qv = (double)mv.x, uv = (double)mv.y;
-            acc.v[0] = fma(wsum, fma(qv, c2d, uv * s2d), acc.v[0]);
-            acc.v[1] = fma(wsum, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
+        }
 #pragma unroll
-            for (int k = 2; k < CS; ++k)
-                if (k < a.nslots) acc.v[k] = fma(wsum, (double)slot(mv, k), acc.v[k]);
-        } else {
+        for (int u = 0; u < kP2Cells; ++u) {
+            if (wsum[u] == 0.0) continue;
+            const int r = rr[u], pos = pp[u];
+            const double wsum_u = wsum[u];
+            const float4 mv_u = mv[u];
+            if constexpr (QU) {
+                double c2d, s2d;
+                // the half longitude gap relative to lam (a difference of angles)
+                const double hd = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
+                const float st2 = R.st2(r);
+                if (ir_lo + r == sp.coinc_ring && pos == sp.coinc_pos) {  // tod_bilinear.py:108-109
+                    c2d = 1.0;
+                    s2d = 0.0;
+                } else if (st2 < kSpinF32St2 && fabs(hd) < 0.05) {
+                    // pixel within ~2 deg of the boresight: binary32
+                    float sb, cb, cf, sf;
+                    sincos_any_f((float)hd, sb, cb);
+                    const float sha = (float)sp.sha, cha = (float)sp.cha;
+                    Spin2RingF g;
+                    g.st2 = st2;
+                    g.S = R.sS(r);
+                    g.s2dt = R.s2dt(r);
+                    g.zsum = R.zsum(r);
+                    g.SC4 = R.sc4(r);
+                    spin2_from_half_f(g, fmaf(sha, cb, -(cha * sb)), fmaf(cha, cb, sha * sb), cf, sf);
+                    c2d = (double)cf;
+                    s2d = (double)sf;
+                } else {
+                    double sb, cb;
+                    sincos_any(hd, sb, cb);
+                    const double sh = fma(sp.sha, cb, -(sp.cha * sb));
+                    const double ch = fma(sp.cha, cb, sp.sha * sb);
+                    const Spin2Ring g = spin2_ring(R.z(r), R.sth(r), sp.z_pts, sp.sth_pts);
+                    spin2_from_half(g, sh, ch, c2d, s2d);
+                }
+                const double qv = (double)mv_u.x, uv = (double)mv_u.y;
+                acc.v[0] = fma(wsum_u, fma(qv, c2d, uv * s2d), acc.v[0]);
+                acc.v[1] = fma(wsum_u, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
 #pragma unroll
-            for (int k = 0; k < CS; ++k)
-                if (k < a.nslots) acc.v[k] = fma(wsum, (double)slot(mv, k), acc.v[k]);
+                for (int k = 2; k < CS; ++k)
+                    if (k < a.nslots) acc.v[k] = fma(wsum_u, (double)slot(mv_u, k), acc.v[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < CS; ++k)
+                    if (k < a.nslots) acc.v[k] = fma(wsum_u, (double)slot(mv_u, k), acc.v[k]);
+            }
         }
     }
     return acc;
