@@ -1835,6 +1835,24 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
     return acc;
 }
 
+// The wave's sums of the four slots into red[0..3]: a butterfly that
+// halves the values per lane at each step (reduce-scatter: 7 exchanges of a
+// double instead of 4 x 6), leaving slot 2 (lane >= 32) + ((lane >> 4) & 1)
+// summed over each group of 16 lanes.
+__device__ __forceinline__ void fp_wave_reduce4(const Acc4& acc, double* red) {
+    const int lane = threadIdx.x & 63;
+    const bool hi = lane >= 32;
+    const double k0 = hi ? acc.v[2] : acc.v[0], k1 = hi ? acc.v[3] : acc.v[1];
+    const double s0 = hi ? acc.v[0] : acc.v[2], s1 = hi ? acc.v[1] : acc.v[3];
+    const double a0 = k0 + __shfl_xor(s0, 32, 64);
+    const double a1 = k1 + __shfl_xor(s1, 32, 64);
+    const bool b4 = (lane >> 4) & 1;
+    double v = (b4 ? a1 : a0) + __shfl_xor(b4 ? a0 : a1, 16, 64);
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((lane & 15) == 0) red[(hi ? 2 : 0) + (b4 ? 1 : 0)] = v;
+}
+
 // Thread 0: add a finished sample's block sums (red[], complete since the
 // last barrier) to its tod entries, slots in order (two slots may share a
 // row when c_q == c_u).  Each tod entry belongs to one sample and each sample
@@ -1905,13 +1923,7 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     // wave reads between the barrier above and the one below
     if constexpr (MODE == 3)
         if (have_next) fp_load_values<CS>(an, fp_frame(buf ^ 1), fp_rings(buf ^ 1));
-#pragma unroll
-    for (int k = 0; k < kMaxSlots; ++k) {
-        if (k < CS) {
-            const double v = wave_sum(acc.v[k]);
-            if (lane == 0) fp_red()[wave * kMaxSlots + k] = v;
-        }
-    }
+    fp_wave_reduce4(acc, fp_red() + wave * kMaxSlots);
     __syncthreads();
     fp_flush(an, fp_job(buf));
 }
