@@ -212,14 +212,17 @@ int todg_corr_adam(float *params, const float *grad, float *m, float *v, int ste
  * per pixel by the reference's spin-2 angle, tod_bilinear.py:403-432).
  * Where the corrector does not apply (the rings next to the poles) or its
  * correction exceeds `bound` (in units of the neighbourhood RMS; INFINITY:
- * no bound) the plain bilinear weights are used.  stats (optional, 2
- * counters, accumulated): corrected and plain beam-pixel samples. */
+ * no bound) the plain bilinear weights are used, and so they are for beam
+ * pixels n_corrected .. S-1 (callers that correct only the strongest beam
+ * pixels sort the beam by value first; n_corrected >= S: all).  stats
+ * (optional, 2 counters, accumulated): corrected and plain beam-pixel
+ * samples. */
 int todg_gather_accum_learned(const float *vec_orig, const float *axes, const float *cos_a,
                               const float *sin_a, const float *ax_pts, const float *cos_p,
                               const float *sin_p, int64_t nside, const float *map_packed, int C,
                               const float *beam_vals, int64_t B, int64_t S, double *tod,
                               int c_q, int c_u, const float *params, float bound,
-                              unsigned long long *stats, void *stream);
+                              int64_t n_corrected, unsigned long long *stats, void *stream);
 
 /* C (M x N) = alpha op(A) op(B) + beta C on BF16 MFMA (row-major float32
  * operands rounded to bfloat16; ta / tb transpose A / B). */

@@ -280,3 +280,35 @@ def test_learned_stats_and_bound(torch):
     ref = np.zeros((3, B))
     O.gather_accum_fused(vec, *params, nside, mp, bv, B, S, ref, 1, 2)
     assert rel_err(got, ref) <= 2e-6
+
+
+def test_learned_power_corrects_only_the_strongest_pixels(torch):
+    """power < 1: the strongest beam pixels holding that power take the
+    corrector and the rest plain bilinear, i.e. the TOD splits into a
+    corrected part and a bilinear part over the two pixel sets."""
+    from cases import rel_err
+    from tod_generation_mapbased_beam_amd import inputs
+    from tod_generation_mapbased_beam_amd import kernels as K
+    from tod_generation_mapbased_beam_amd import tod_rotations as R
+    from tod_generation_mapbased_beam_amd.corrector import gather_accum_learned
+
+    rng = np.random.default_rng(31)
+    nside, B = 64, 40
+    vec, bv, _ = inputs.gaussian_beam(128, 120.0, 0.99)
+    S = vec.shape[0]
+    params = R.rotation_params_for_pointing(*inputs.ring_scan(B, 0.5))
+    mp = rng.standard_normal((3, 12 * nside * nside)).astype(np.float32)
+    flat = M.init_params(3) * 4.0
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    got = np.zeros((3, B))
+    gather_accum_learned(vec, *params, nside, mp, bv, B, S, got, 1, 2, params=flat, power=0.9, stats=stats)
+    order = np.argsort(-np.abs(bv), kind="stable")
+    cum = np.cumsum(np.abs(bv[order]).astype(np.float64))
+    k = int(np.searchsorted(cum, 0.9 * cum[-1])) + 1
+    top, rest = order[:k], order[k:]
+    ref = np.zeros((3, B))
+    gather_accum_learned(vec[top], *params, nside, mp, bv[top], B, k, ref, 1, 2, params=flat)
+    K._gather_accum_fused_jit(vec[rest], *params, nside, mp, bv[rest], B, S - k, ref, 1, 2)
+    assert rel_err(got, ref) <= 1e-9
+    st = stats.cpu().numpy()
+    assert st.sum() == B * S and st[0] <= B * k and st[1] >= B * (S - k)

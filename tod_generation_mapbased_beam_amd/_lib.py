@@ -59,7 +59,7 @@ SIGNATURES = {
     "todg_gather_accum_learned": (
         _i32,
         [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _p,
-         ctypes.c_float, _p, _p],
+         ctypes.c_float, _i64, _p, _p],
     ),
     "todg_corr_gemm_bf16": (_i32, [_i32, _i32, _i32, _p, _i32, _i32, _p, _i32, _i32, _p, _i32,
                                     ctypes.c_float, ctypes.c_float, _p]),

@@ -203,18 +203,23 @@ class Corrector:
 
 def gather_accum_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_stacked,
                          beam_vals, B, S, tod, c_q=-1, c_u=-1, params=None, bound=float("inf"),
-                         stats=None):
+                         stats=None, power=1.0):
     """Beam TOD with the corrector fused into the gather (same arguments as
     ``kernels._gather_accum_fused_jit`` plus the corrector's parameters);
     accumulates into ``tod``.  The corrector reads the first field outside
     the Q/U pair (I) and its corrected weights serve every field (Q/U
     rotated by spin-2 per pixel).  ``bound``: largest |correction| applied,
     in units of the neighbourhood RMS (plain bilinear beyond it).
+    ``power`` < 1: only the strongest beam pixels holding that fraction of
+    the beam's total |value| take the corrector, the weaker tail plain
+    bilinear (the beam is reordered by value for the call).
     ``stats``: optional int64 device tensor (2,) that receives (corrected,
     plain bilinear) beam-pixel-sample counts."""
     from .device import device_map_for
 
     B, S, c_q, c_u = int(B), int(S), int(c_q), int(c_u)
+    if not 0.0 < float(power) <= 1.0:
+        raise ValueError("power must be in (0, 1]")
     dmap = device_map_for(mp_stacked, c_q, c_u)
     if dmap.nside != int(nside):
         raise ValueError(f"map nside {dmap.nside} != nside {nside}")
@@ -228,10 +233,17 @@ def gather_accum_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nsi
     if t.dtype != torch.float64 or tuple(t.shape) != (dmap.C, B):
         raise ValueError(f"tod must be float64 of shape ({dmap.C}, {B})")
     v, ax, ca, sa, bp, cp, sp, bv = ops
+    n_corr = S
+    if power < 1.0:
+        order = torch.argsort(torch.abs(bv.reshape(-1)), descending=True, stable=True)
+        v = v.reshape(-1, 3)[order].contiguous()
+        bv = bv.reshape(-1)[order].contiguous()
+        cum = torch.cumsum(torch.abs(bv).double(), 0)
+        n_corr = int(torch.searchsorted(cum, power * float(cum[-1])).item()) + 1
     _lib.check(_lib.lib().todg_gather_accum_learned(
         _lib.ptr(v), _lib.ptr(ax), _lib.ptr(ca), _lib.ptr(sa), _lib.ptr(bp), _lib.ptr(cp), _lib.ptr(sp),
         int(nside), _lib.ptr(dmap.packed), dmap.C, _lib.ptr(bv), B, S, _lib.ptr(t), c_q, c_u,
-        _lib.ptr(p), float(bound), _lib.ptr(stats), stream_handle()))
+        _lib.ptr(p), float(bound), min(n_corr, S), _lib.ptr(stats), stream_handle()))
     if host:
         tod[...] = t.cpu().numpy()
     return tod

@@ -216,6 +216,7 @@ struct TodArgs {
     unsigned long long* corr_stats;    // [0] corrected, [1] plain bilinear beam-pixel samples, or null
     float corr_bound;                  // largest |correction| applied, in neighbourhood RMS units
     int corr_slot;                     // packed slot whose values drive the corrector
+    int64_t corr_count;                // beam pixels [0, corr_count) take the corrector, the rest bilinear
 };
 
 struct Sample {
@@ -469,7 +470,9 @@ constexpr int kOffRed = kOffTrig + 2 * kLamN * 8;
 constexpr int kOffScratch = kOffRed + kFpWaves * kMaxSlots * 8;
 constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
 constexpr int kOffW = kOffJob + 2 * 64;
-constexpr int kFpSmemBytes = kOffW + kFpCap * 8;
+// weight-grid cell type (beam value x interpolation weight sums)
+typedef double WCell;
+constexpr int kFpSmemBytes = kOffW + kFpCap * (int)sizeof(WCell);
 // mode 3 (learned corrector) adds the driving field's values of the
 // footprint cells, one 64 x 64-byte staging tile per wave and the biases
 constexpr int kOffIv = kFpSmemBytes;
@@ -520,7 +523,7 @@ __device__ __forceinline__ FpFrame* fp_frame(int buf) {
 __device__ __forceinline__ double* fp_trig() { return &lds_at<double>(kOffTrig); }
 __device__ __forceinline__ double* fp_red() { return &lds_at<double>(kOffRed); }
 __device__ __forceinline__ double* fp_scratch() { return &lds_at<double>(kOffScratch); }
-__device__ __forceinline__ double* fp_W() { return &lds_at<double>(kOffW); }
+__device__ __forceinline__ WCell* fp_W() { return &lds_at<WCell>(kOffW); }
 
 // The per-sample operands a stage needs after the sample's layout: the
 // beam, the tod destination, the sample index and phase 1's claim counter
@@ -653,7 +656,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
                                             float bvf) {
     const FpRings R = fp_rings(buf);
     const FpFrame* fr = fp_frame(buf);
-    double* W = fp_W();
+    WCell* W = fp_W();
     Sample q;
     frame_sample(fr, q);
     const FpSpin sp = fp_spin_of(fr);
@@ -670,7 +673,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
             const int lr = o.ring[k] - ir_lo;
             const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
             if (li >= 0) {
-                atomicAdd(&W[li], o.w[k] * bv);
+                atomicAdd(&W[li], (WCell)(o.w[k] * bv));
             } else {
                 const Acc4 d = fp_direct<QU, CS>(a, sp, o.ring[k], o.pos[k], o.w[k] * bv);
                 for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
@@ -683,7 +686,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
         const int lr = ring - ir_lo;
         const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
         if (li >= 0)
-            atomicAdd(&W[li], bv);
+            atomicAdd(&W[li], (WCell)bv);
         else
             acc = fp_direct<QU, CS>(a, sp, ring, pos, bv);
     } else {
@@ -696,7 +699,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
                 const int lr = ring - ir_lo;
                 const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
                 if (li >= 0) {
-                    atomicAdd(&W[li], w * scale);
+                    atomicAdd(&W[li], (WCell)(w * scale));
                 } else {
                     const Acc4 d = fp_direct<QU, CS>(a, sp, ring, pos, w * scale);
                     for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
@@ -709,7 +712,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
                 const int lr = o.ring[k] - ir_lo;
                 const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
                 if (li >= 0) {
-                    atomicAdd(&W[li], o.w[k] * bv);
+                    atomicAdd(&W[li], (WCell)(o.w[k] * bv));
                 } else {
                     const Acc4 d = fp_direct<QU, CS>(a, sp, o.ring[k], o.pos[k], o.w[k] * bv);
                     for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
@@ -1012,7 +1015,7 @@ __device__ __forceinline__ double sqrt_fast(double x) {
 // or slots outside the laid-out windows, large longitude offsets -- goes to
 // the exact path.  Weights only need tolerance accuracy.
 __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, int nside,
-                                                 int ir_lo, int nr, const FpRings& R, double* W,
+                                                 int ir_lo, int nr, const FpRings& R, WCell* W,
                                                  double u0, double u1, double u2, double bv) {
     // (conditions are combined with & rather than && so that they compile to
     // masks instead of branches; every LDS index is clamped into range first)
@@ -1065,12 +1068,12 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     const float wb = fmaf(xf * x2, pp, xf) * (float)R.inv_dth(la) * bvf;
     const float wa = bvf - wb;
     const float faf = (float)fa, fbf = (float)fb;
-    double* ca = W + R.off(la) + ia;
-    double* cb = W + R.off(la + 1) + ib;
-    atomicAdd(ca, (double)(wa - wa * faf));
-    atomicAdd(ca + 1, (double)(wa * faf));
-    atomicAdd(cb, (double)(wb - wb * fbf));
-    atomicAdd(cb + 1, (double)(wb * fbf));
+    WCell* ca = W + R.off(la) + ia;
+    WCell* cb = W + R.off(la + 1) + ib;
+    atomicAdd(ca, (WCell)(wa - wa * faf));
+    atomicAdd(ca + 1, (WCell)(wa * faf));
+    atomicAdd(cb, (WCell)(wb - wb * fbf));
+    atomicAdd(cb + 1, (WCell)(wb * fbf));
     return true;
 }
 
@@ -1097,7 +1100,7 @@ __device__ __forceinline__ double cos_tiny(double x) {
 constexpr double kCertNearest = 1e-12;
 
 __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, int nside, int ir_lo,
-                                                int nr, const FpRings& R, double* W, double u0,
+                                                int nr, const FpRings& R, WCell* W, double u0,
                                                 double u1, double u2, double bv) {
     const double xp = fma(m[0], u0, fma(m[1], u1, m[2] * u2));
     const double yp = fma(m[3], u0, fma(m[4], u1, m[5] * u2));
@@ -1153,7 +1156,7 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
     }
     ok &= best - second > kCertNearest;
     if (!ok) return false;
-    atomicAdd(W + cell, bv);
+    atomicAdd(W + cell, (WCell)bv);
     return true;
 }
 
@@ -1165,7 +1168,7 @@ constexpr int kClaim = 256;  // beam pixels per claim (4 per lane)
 
 template <int MODE, bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFrame* fr,
-                                          const FpRings& R, int buf, double* W) {
+                                          const FpRings& R, int buf, WCell* W) {
     const int lane = threadIdx.x & 63;
     const int nside = a.nside;
     const gptr<const float> __restrict__ vec = global_in(J->vec_orig);
@@ -1531,7 +1534,7 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
     if (!cw.apply) return fp_exact_pixel<0, QU, CS>(a, buf, u0, u1, u2, bvf);
     const FpRings R = fp_rings(buf);
     const FpFrame* fr = fp_frame(buf);
-    double* W = fp_W();
+    WCell* W = fp_W();
     Sample q;
     frame_sample(fr, q);
     const FpSpin sp = fp_spin_of(fr);
@@ -1561,7 +1564,7 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
             const int lr = ring[r] - ir_lo;
             const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
             if (li >= 0) {
-                atomicAdd(&W[li], w);
+                atomicAdd(&W[li], (WCell)w);
             } else {
                 const Acc4 d = fp_direct<QU, CS>(a, sp, ring[r], pos, w);
                 for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
@@ -1581,7 +1584,7 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
 // the bound, the plain bilinear weights are used.
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const FpFrame* fr,
-                                               const FpRings& R, int buf, double* W,
+                                               const FpRings& R, int buf, WCell* W,
                                                const CorrRegs& C, unsigned long long& n_corr,
                                                unsigned long long& n_fb) {
     const int lane = threadIdx.x & 63;
@@ -1595,6 +1598,7 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
     const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     const float bound = a.corr_bound;
+    const int64_t K = a.corr_count;
     const float* Iv = fp_Iv();
     double m[9];
 #pragma unroll
@@ -1611,6 +1615,24 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
             it = 0;
         }
         if (s0 + it >= S) break;  // no lane has a beam pixel left (uniform)
+        if (s0 >= K) {
+            // the whole claim lies beyond the corrected beam pixels: the
+            // plain bilinear path, no MLP (uniform branch)
+            for (; it < kPer; ++it) {
+                const int64_t s = s0 + (int64_t)lane * kPer + it;
+                if (s < S) {
+                    const float u0 = vec[3 * s], u1 = vec[3 * s + 1], u2 = vec[3 * s + 2], bvf = bvs[s];
+                    ++n_fb;
+                    if (!fp_fast_bilinear(m, nside, ir_lo, nr, R, W, (double)u0, (double)u1, (double)u2,
+                                          (double)bvf)) {
+                        const Acc4 d = fp_exact_pixel<0, QU, CS>(fp_globals(a), buf, u0, u1, u2, bvf);
+#pragma unroll
+                        for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
+                    }
+                }
+            }
+            continue;
+        }
         // lanes kPer beam pixels apart (fewer same-cell LDS atomics)
         const int64_t s = s0 + (int64_t)lane * kPer + it;
         ++it;
@@ -1650,7 +1672,7 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
                 for (int k = 0; k < 16; ++k) val[k] = ci.val[k];
             }
         }
-        const bool on = kind == 1 || kind == 2;
+        const bool on = (kind == 1 || kind == 2) && s < K;
         float mean = 0.f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) mean += val[k];
@@ -1693,12 +1715,12 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
-                    for (int t = 0; t < 4; ++t) atomicAdd(W + G.cell[r] + t, w16[4 * r + t]);
+                    for (int t = 0; t < 4; ++t) atomicAdd(W + G.cell[r] + t, (WCell)w16[4 * r + t]);
             } else {
-                atomicAdd(W + G.cell[1] + 1, w16[5]);
-                atomicAdd(W + G.cell[1] + 2, w16[6]);
-                atomicAdd(W + G.cell[2] + 1, w16[9]);
-                atomicAdd(W + G.cell[2] + 2, w16[10]);
+                atomicAdd(W + G.cell[1] + 1, (WCell)w16[5]);
+                atomicAdd(W + G.cell[1] + 2, (WCell)w16[6]);
+                atomicAdd(W + G.cell[2] + 1, (WCell)w16[9]);
+                atomicAdd(W + G.cell[2] + 2, (WCell)w16[10]);
             }
         } else if (kind >= 2) {
             CorrW cw;
@@ -1748,7 +1770,7 @@ constexpr int kP2Cells = 4;
 
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
-                                          double* W, Acc4 acc) {
+                                          WCell* W, Acc4 acc) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
@@ -1764,7 +1786,7 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
         for (int u = 0; u < kP2Cells; ++u) {
             const int c0 = base + u * kFpThreads;
             const int j = c0 + lane;
-            wsum[u] = j < total ? W[j] : 0.0;
+            wsum[u] = j < total ? (double)W[j] : 0.0;
             if (c0 < total) r0 = fp_ring_of(R, nr, r0, c0);  // uniform
             rr[u] = r0;
         }
@@ -1773,7 +1795,7 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
             const int j = base + u * kFpThreads + lane;
             pp[u] = 0;
             if (wsum[u] != 0.0) {
-                W[j] = 0.0;
+                W[j] = (WCell)0;
                 const int r = fp_ring_of(R, nr, rr[u], j);
                 int pos = R.i0(r) + (j - R.off(r));
                 const int n = R.n(r);
@@ -1872,7 +1894,7 @@ __device__ __forceinline__ void fp_flush(const TodArgs& a, const FpJob* J) {
 
 // Zero the weight grid and copy the reference-longitude table into LDS.
 __device__ __forceinline__ void fp_init(const double* lam_trig) {
-    for (int j = threadIdx.x; j < kFpCap; j += kFpThreads) fp_W()[j] = 0.0;
+    for (int j = threadIdx.x; j < kFpCap; j += kFpThreads) fp_W()[j] = (WCell)0;
     for (int j = threadIdx.x; j < 2 * kLamN; j += kFpThreads) fp_trig()[j] = lam_trig[j];
     __syncthreads();
 }
@@ -2405,6 +2427,7 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
     a->corr_stats = nullptr;
     a->corr_bound = 0.f;
     a->corr_slot = (*qu && nslots > 2) ? 2 : 0;  // the first field outside the Q/U pair
+    a->corr_count = 0;
     *CS = slots_stride(nslots);
     return TODG_OK;
 }
@@ -2432,7 +2455,7 @@ int run_learned(const float* vec_orig, const float* axes, const float* cos_a, co
                 const float* ax_pts, const float* cos_p, const float* sin_p, int64_t nside,
                 const float* map_packed, int C, const float* beam_vals, int64_t B, int64_t S,
                 double* tod, int c_q, int c_u, const float* params, float bound,
-                unsigned long long* stats, void* stream) {
+                int64_t n_corrected, unsigned long long* stats, void* stream) {
     TodArgs a;
     bool qu;
     int CS;
@@ -2442,9 +2465,11 @@ int run_learned(const float* vec_orig, const float* axes, const float* cos_a, co
     if (!(bound >= 0.f)) return fail(TODG_ERR_ARG, "bound must be >= 0 (inf: no bound)");
     if (B == 0 || S == 0) return TODG_OK;
     if (!params) return fail(TODG_ERR_ARG, "null corrector parameters");
+    if (n_corrected < 0) return fail(TODG_ERR_ARG, "n_corrected must be >= 0");
     a.corr = params;
     a.corr_stats = stats;
     a.corr_bound = bound;
+    a.corr_count = n_corrected < S ? n_corrected : S;
     hipError_t e = launch_mode<3>(a, qu, CS, 0, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TODG_OK;
@@ -2542,10 +2567,10 @@ int todg_gather_accum_learned(const float* vec_orig, const float* axes, const fl
                               const float* sin_a, const float* ax_pts, const float* cos_p,
                               const float* sin_p, int64_t nside, const float* map_packed, int C,
                               const float* beam_vals, int64_t B, int64_t S, double* tod, int c_q,
-                              int c_u, const float* params, float bound,
+                              int c_u, const float* params, float bound, int64_t n_corrected,
                               unsigned long long* stats, void* stream) {
     return run_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
-                       beam_vals, B, S, tod, c_q, c_u, params, bound, stats, stream);
+                       beam_vals, B, S, tod, c_q, c_u, params, bound, n_corrected, stats, stream);
 }
 
 int todg_interp_points(int mode, int64_t nside, const float* map, int64_t N, const double* z,

@@ -87,7 +87,7 @@ def _time(fn, reps=2):
 
 
 def evaluate(c: C.Corrector, nside=2048, n_det=8, n_samples=1_728_000, fwhm=30.0, bound=4.0,
-             lmax=None, seed=7, n_rot=32, n_rot_points=64):
+             lmax=None, seed=7, n_rot=32, n_rot_points=64, powers=(0.99, 0.95, 0.9)):
     dev = device()
     lmax = 3 * nside - 1 if lmax is None else lmax
     t0 = time.time()
@@ -100,6 +100,8 @@ def evaluate(c: C.Corrector, nside=2048, n_det=8, n_samples=1_728_000, fwhm=30.0
            "bound": bound}
     res = {"bilinear": [], "learned": [], "bicubic": []}
     times = {"bilinear": 0.0, "learned": 0.0, "bicubic": 0.0}
+    pres = {pw: [] for pw in powers}
+    ptimes = {pw: 0.0 for pw in powers}
     stats = torch.zeros(2, dtype=torch.int64, device=dev)
     for d in range(n_det):
         th, ph, ps = inputs.ring_scan(n_samples, 20.0, t0=3600.0 * d)
@@ -122,6 +124,16 @@ def evaluate(c: C.Corrector, nside=2048, n_det=8, n_samples=1_728_000, fwhm=30.0
 
         for k in res:
             times[k] += _time(lambda: run(k), reps=1)
+        for pw in powers:  # the corrector on the strongest beam pixels only
+            tp = torch.zeros((3, n_samples), dtype=torch.float64, device=dev)
+
+            def run_p():
+                tp.zero_()
+                C.gather_accum_learned(v, *params, nside, dmap, b, n_samples, S, tp, 1, 2,
+                                       params=c.params, bound=bound, power=pw)
+
+            ptimes[pw] += _time(run_p, reps=1)
+            pres[pw].append(tp)
         C.gather_accum_learned(v, *params, nside, dmap, b, n_samples, S,
                                torch.zeros((3, n_samples), dtype=torch.float64, device=dev), 1, 2,
                                params=c.params, bound=bound, stats=stats)
@@ -141,6 +153,14 @@ def evaluate(c: C.Corrector, nside=2048, n_det=8, n_samples=1_728_000, fwhm=30.0
             "kernel_seconds": times[k],
         }
     out["bicubic"] = {"kernel_seconds": times["bicubic"]}
+    rb0 = torch.sqrt(torch.mean((torch.cat(res["bilinear"], dim=1) - judge) ** 2, dim=1))
+    out["learned_power_sweep"] = []
+    for pw in powers:
+        rp = torch.sqrt(torch.mean((torch.cat(pres[pw], dim=1) - judge) ** 2, dim=1))
+        out["learned_power_sweep"].append({
+            "power": pw, "kernel_seconds": ptimes[pw],
+            "time_ratio_over_bilinear": ptimes[pw] / times["bilinear"],
+            "rms_ratio_over_bilinear": (rp / rb0).tolist()})
     rb = np.array(out["bilinear"]["rms_residual_vs_bicubic"])
     rl = np.array(out["learned"]["rms_residual_vs_bicubic"])
     out["rms_ratio_learned_over_bilinear"] = (rl / rb).tolist()
