@@ -786,8 +786,11 @@ __device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* r
         cosr = fmax(-1.0, fmin(1.0, mn));
     }
     // Disc radius for the ring range and the per-ring longitude half-width:
-    // beam radius + margin_px pixel sizes (bilinear neighbours lie within
-    // ~1.5 of the query, bicubic candidates within ~4) + a rounding allowance.
+    // beam radius + margin_px pixel sizes + a rounding allowance.  The
+    // margin covers the neighbour rings (one ring spacing, at most 0.9 pixel
+    // sizes, from the query for bilinear and nearest; three spacings and the
+    // stencil's slots for bicubic); the slot windows add the longitude
+    // neighbours.
     BeamFrame f;
     f.cx = cx;
     f.cy = cy;
@@ -801,8 +804,10 @@ __device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* r
 // slot that the rotated beam's bilinear neighbours can touch, as per-ring
 // slot windows packed into the weight grid.  The windows only steer work
 // (a neighbour outside them takes the exact path), so they are estimated in
-// binary32 with a margin of three slots per side; the reference longitude
-// lam and everything the index path derives from it are binary64.
+// binary32: on each ring the disc's chord [L, R] in slot coordinates, as
+// slots floor(L) - 1 .. at least floor(R) + 2 (the right neighbour of the
+// last slot and one more); the reference longitude lam and everything the
+// index path derives from it are binary64.
 __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const BeamFrame& bf,
                                           const double* trig, FpFrame* fr,
                                           FpRings R) {
@@ -878,12 +883,12 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
             } else if (cd < 1.f) {
                 const float dp = acosf(cd);
                 const float spr = nf * (float)(1.0 / kTwoPi);  // slots per radian
-                const float wf = 2.f * dp * spr + 6.f;
+                const float wf = 2.f * dp * spr + 5.f;
                 if (wf >= nf) {
                     full = true;
                 } else {
                     width = (int)wf;
-                    const int f = (int)floorf((fphf - dp - (float)g.phi0) * spr) - 2;
+                    const int f = (int)floorf((fphf - dp - (float)g.phi0) * spr) - 1;
                     base = tc - (double)f;
                     const int fm = f % g.n;
                     i0 = fm < 0 ? fm + g.n : fm;
@@ -1766,7 +1771,7 @@ __device__ __forceinline__ int fp_ring_of(const FpRings& R, int nr, int r0, int 
 // ring were computed with the layout), kP2Cells such chunks per round, so
 // that each lane has kP2Cells independent weight reads and gathers in
 // flight.  Returns the per-thread sums (outside-grid sums included).
-constexpr int kP2Cells = 4;
+constexpr int kP2Cells = 5;
 
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
@@ -1965,10 +1970,10 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
     if (bbeg + kx >= bend) return;  // uniform over the block
     fp_init(a.lam_trig);
-    // footprint margin: bilinear neighbours lie within ~1.5 pixel sizes of
-    // the query, 4 x 4 corrector neighbourhoods within ~3.6, bicubic
-    // candidates within ~4
-    const double margin = MODE == 2 ? 5.0 : (MODE == 3 ? 4.5 : 2.0);
+    // footprint margin in pixel sizes: bilinear and nearest neighbour rings
+    // lie within one ring spacing (<= 0.9) of the query, 4 x 4 corrector
+    // neighbourhoods within ~3.6, bicubic candidates within ~4
+    const double margin = MODE == 2 ? 5.0 : (MODE == 3 ? 4.5 : 1.25);
     const BeamFrame bf = fp_beam_frame(beam_src(a), bbeg + kx, fp_scratch(), margin);
     int64_t b = bbeg + kx;
     if ((threadIdx.x >> 6) == 0) {
@@ -2136,7 +2141,7 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
     fp_init(a.lam_trig);
     BatchCursor c = {0, 0, 0, -1, false, {0.0, 0.0, 1.0, 0.0, 1.0}};
     int64_t b = 0;
-    const double margin_px = MODE == 2 ? 5.0 : 2.0;
+    const double margin_px = MODE == 2 ? 5.0 : 1.25;
     if (!batch_next(ba, c, a, b, margin_px)) return;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
