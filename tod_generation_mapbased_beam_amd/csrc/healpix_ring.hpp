@@ -191,6 +191,14 @@ __device__ __forceinline__ void sincos_half(double x, double& s, double& c) {
     }
 }
 
+// 1/x from the hardware estimate plus one Newton step: the estimate's
+// relative error e is below 2^-23 (measured 2^-24.4 on gfx950), the step's
+// below e^2 + 2^-52 < 2e-14 (measured 2^-48.7).
+__device__ __forceinline__ double rcp_nr1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
 // 1/x from the hardware estimate plus two Newton steps (< 1 ulp error).
 __device__ __forceinline__ double rcp_nr(double x) {
     double r = __builtin_amdgcn_rcp(x);

@@ -427,7 +427,7 @@ struct alignas(16) FpFrame {
     double sha, cha;  // sin, cos of half the boresight longitude minus lam
     int ir_lo, nr, total;
     int coinc_ring, coinc_pos;  // the map pixel whose centre IS the boresight, or -1
-    int pad;
+    int coinc_cell;             // its weight-grid cell, or -1
 };
 
 // Map and geometry operands of the exact path (passed by value to the
@@ -460,19 +460,21 @@ extern __shared__ __align__(16) unsigned char fp_smem[];
 
 constexpr int kFpCap = 4608;       // weight-grid cells
 constexpr int kFpMaxRings = 128;   // ring descriptors per buffer
-constexpr int kRingD = 12;        // double arrays per ring buffer
+constexpr int kRingD = 14;        // 8-byte arrays per ring buffer
 constexpr int kRingI = 5;         // int arrays per ring buffer
 constexpr int kRingBuf = kFpMaxRings * (kRingD * 8 + kRingI * 4);
 constexpr int kOffRings = 0;
 constexpr int kOffFrame = kOffRings + 2 * kRingBuf;
-constexpr int kOffTrig = kOffFrame + 2 * (int)sizeof(FpFrame);
-constexpr int kOffRed = kOffTrig + 2 * kLamN * 8;
+constexpr int kOffRed = kOffFrame + 2 * (int)sizeof(FpFrame);
 constexpr int kOffScratch = kOffRed + kFpWaves * kMaxSlots * 8;
 constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
 constexpr int kOffW = kOffJob + 2 * 64;
 // weight-grid cell type (beam value x interpolation weight sums)
 typedef double WCell;
-constexpr int kFpSmemBytes = kOffW + kFpCap * (int)sizeof(WCell);
+// ring of each weight-grid cell (footprint-relative, < kFpMaxRings <= 256),
+// one table per buffer
+constexpr int kOffRtab = kOffW + kFpCap * (int)sizeof(WCell);
+constexpr int kFpSmemBytes = kOffRtab + 2 * kFpCap;
 // mode 3 (learned corrector) adds the driving field's values of the
 // footprint cells, one 64 x 64-byte staging tile per wave and the biases
 constexpr int kOffIv = kFpSmemBytes;
@@ -482,6 +484,7 @@ constexpr int kOffBias = kOffStage + kFpWaves * kStageBytes;  // b2 (64), b3 (16
 constexpr int kFpSmemBytesCorr = kOffBias + 80 * 4;
 static_assert(kOffStage % 16 == 0 && kOffBias % 16 == 0, "16-byte LDS vectors");
 static_assert(kFpSmemBytesCorr <= 160 * 1024, "one workgroup per CU");
+static_assert(kFpSmemBytes <= 80 * 1024, "two workgroups per CU");
 
 template <typename T>
 __device__ __forceinline__ T& lds_at(unsigned off) {
@@ -509,6 +512,21 @@ struct FpRings {
     __device__ float& s2dt(int r) const { return f(9, r); }
     __device__ float& zsum(int r) const { return f(10, r); }
     __device__ float& sc4(int r) const { return f(11, r); }
+    // more binary32 values in the second halves of those slots:
+    // 1 / (theta_below - theta_above), and the sweep's spin-2 longitudes --
+    // half the longitude gap between window slot k and the boresight is
+    // (k - kb) hdp + gb, kb the window slot nearest the boresight
+    __device__ float& g(int k, int r) const { return lds_at<float>(o + 8 * (k * kFpMaxRings + r) + 4); }
+    __device__ float& inv_dth_f(int r) const { return g(7, r); }
+    __device__ float& gb(int r) const { return g(8, r); }
+    __device__ float& hdp(int r) const { return g(9, r); }
+    __device__ float& kb(int r) const { return g(10, r); }
+    // base + off: lam's weight-grid cell coordinate (binary64; off < 2^13, so
+    // the sum rounds by < 5e-13)
+    __device__ double& baseo(int r) const { return d(12, r); }
+    // {off, max(width - 1, 0)}: a bilinear pair (c, c + 1) lies in the
+    // window iff (unsigned)(c - off) < max(width - 1, 0)
+    __device__ int2& ow(int r) const { return lds_at<int2>(o + 8 * (13 * kFpMaxRings + r)); }
     __device__ int& width(int r) const { return i(0, r); }
     __device__ int& off(int r) const { return i(1, r); }    // non-decreasing in r
     __device__ int& i0(int r) const { return i(2, r); }
@@ -520,7 +538,7 @@ __device__ __forceinline__ FpRings fp_rings(int buf) { return FpRings{(unsigned)
 __device__ __forceinline__ FpFrame* fp_frame(int buf) {
     return &lds_at<FpFrame>(kOffFrame + buf * (int)sizeof(FpFrame));
 }
-__device__ __forceinline__ double* fp_trig() { return &lds_at<double>(kOffTrig); }
+__device__ __forceinline__ unsigned char* fp_rtab(int buf) { return fp_smem + kOffRtab + buf * kFpCap; }
 __device__ __forceinline__ double* fp_red() { return &lds_at<double>(kOffRed); }
 __device__ __forceinline__ double* fp_scratch() { return &lds_at<double>(kOffScratch); }
 __device__ __forceinline__ WCell* fp_W() { return &lds_at<WCell>(kOffW); }
@@ -809,8 +827,7 @@ __device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* r
 // last slot and one more); the reference longitude lam and everything the
 // index path derives from it are binary64.
 __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const BeamFrame& bf,
-                                          const double* trig, FpFrame* fr,
-                                          FpRings R) {
+                                          FpFrame* fr, FpRings R, unsigned char* rtab) {
     const int lane = threadIdx.x & 63;
     const int nside = a.nside;
     Sample q;
@@ -840,6 +857,7 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
     int jl = (int)rintf(fphf * (float)(kLamN / kTwoPi));
     jl = jl >= kLamN ? jl - kLamN : jl;
     const double lam = (double)jl * (kTwoPi / kLamN);
+    const gptr<const double> trig = global_in(a.lam_trig);
     const double cl = trig[2 * jl], sl = trig[2 * jl + 1];
 
     const float Rf = (float)bf.R, cosRf = (float)bf.cosR;
@@ -866,17 +884,22 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
     // boresight z and sin(theta) (tod_bilinear.py:306-310)
     const double z_pts = fmax(-1.0, fmin(1.0, q.bz));
     const double sth_pts = sqrt(fmax(0.0, 1.0 - q.bz * q.bz));
+    // the boresight relative to lam: (x, y) = Rz(-lam) (bx, by), longitude
+    // offset phb = atan2(y, x) (0 on the axis, as the reference's atan2(0, 0))
+    const double xb = fma(cl, q.bx, sl * q.by);
+    const double yb = fma(cl, q.by, -(sl * q.bx));
+    const double phb = atan2(yb, xb);
     int carry = 0, laid = 0;
     for (int r0 = 0; r0 < nr; r0 += 64) {
         const int r = r0 + lane;
         int width = 0, i0 = 0;
+        double base = 0.0;
         if (r < nr) {
             const RingEntry& g = a.rings[ir_lo + r];
             const float nf = (float)g.n;
             const double tc = (lam - g.phi0) * g.inv_dphi;  // lam's slot coordinate
             const float den = fstf * (float)g.sth;
             const float cd = den > 1e-6f ? (cosRf - fzf * (float)g.z) / den : -2.f;
-            double base = 0.0;
             bool full = false;
             if (cd <= -1.f) {
                 full = true;
@@ -906,9 +929,16 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
             R.sc4(r) = (float)sr.SC4;
             R.base(r) = base;
             R.inv_dphi(r) = g.inv_dphi;
+            // the sweep's half longitude gaps to the boresight, window slot kb
+            // nearest it (window slots are unwrapped: a gap never jumps by pi)
+            const double kbd = rint(fma(phb, g.inv_dphi, base));
+            R.gb(r) = (float)(0.5 * fma(kbd - base, g.dphi, -phb));
+            R.hdp(r) = (float)(0.5 * g.dphi);
+            R.kb(r) = (float)kbd;
             R.z(r) = g.z;
             R.sth(r) = g.sth;
             R.inv_dth(r) = g.inv_dth;
+            R.inv_dth_f(r) = (float)g.inv_dth;
             R.phi0(r) = g.phi0;
             R.dphi(r) = g.dphi;
             R.n(r) = g.n;
@@ -927,6 +957,10 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
             R.i0(r) = i0;
             R.width(r) = fits ? width : 0;  // beyond capacity: accumulated directly
             R.off(r) = off;
+            R.baseo(r) = base + (double)off;
+            R.ow(r) = make_int2(off, fits && width > 1 ? width - 1 : 0);
+            if (fits)
+                for (int k = 0; k < width; ++k) rtab[off + k] = (unsigned char)r;
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
@@ -981,8 +1015,16 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
                 }
             }
         }
+        int cc = -1;  // its weight-grid cell
+        if (cr >= ir_lo && cr - ir_lo < nr) {
+            const int r = cr - ir_lo;
+            int d = cp - R.i0(r);
+            if (d < 0) d += R.n(r);
+            if (d < R.width(r)) cc = R.off(r) + d;
+        }
         fr->coinc_ring = cr;
         fr->coinc_pos = cp;
+        fr->coinc_cell = cc;
         fr->lam = lam;
         fr->z_pts = z_pts;
         fr->sth_pts = sth_pts;
@@ -1016,12 +1058,18 @@ __device__ __forceinline__ double sqrt_fast(double x) {
 // < 1e-11 (including the reference's own rounding of its division), so a
 // coordinate whose fractional part lies in [eps, 1 - eps] has the
 // reference's integer part: ring_above, the two ring slots and hence all four
-// pixels are the reference's.  Everything else -- margins, pole caps, rings
-// or slots outside the laid-out windows, large longitude offsets -- goes to
-// the exact path.  Weights only need tolerance accuracy.
+// pixels are the reference's.  The belt/cap choice needs no test of its own:
+// z can fall on the other side of |z| = 2/3 from the reference's only within
+// ~1e-15 of it, where both ring formulas give nside to within 2e-11, which the
+// certification rejects.  Everything else -- margins, pole caps, rings or
+// slots outside the laid-out windows, large longitude offsets -- goes to the
+// exact path.  Weights only need tolerance accuracy; they are rounded to
+// binary32 before the binary64 atomics, so that a cell's sum is exact, and
+// independent of the order of the adds, while its addends span less than a
+// factor 2^25.
 __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, int nside,
                                                  int ir_lo, int nr, const FpRings& R, WCell* W,
-                                                 double u0, double u1, double u2, double bv) {
+                                                 double u0, double u1, double u2, float bvf) {
     // (conditions are combined with & rather than && so that they compile to
     // masks instead of branches; every LDS index is clamped into range first)
     const double xp = fma(m[0], u0, fma(m[1], u1, m[2] * u2));
@@ -1044,18 +1092,26 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
         const double fr = tr - (double)it;
         ok = (fr > kCertPolar) & (fr < 1.0 - kCertPolar);
     }
-    ok &= fabs(az - kTwoThirds) > 1e-12;
+    // rings la and la + 1 inside the footprint (ir_lo >= 1 and
+    // ir_lo + nr <= 4 nside, so then also 1 <= ira <= 4 nside - 2; nr = 0
+    // when there is no layout)
     int la = ira - ir_lo;
-    ok &= (ira >= 1) & (ira <= 4 * nside - 2) & (la >= 0) & (la + 1 < nr);
-    ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
+    ok &= (unsigned)la < (unsigned)(nr > 1 ? nr - 1 : 0);
+    // longitude offset from lam: atan(t), |t| < kMaxTan (t's relative error
+    // < 2e-14 moves a slot coordinate by < 1e-12)
+    const double t = yp * rcp_nr1(xp);
+    ok &= (xp > 0.0) & (fabs(t) < kMaxTan);
     la = ok ? la : 0;
-    // longitude offset from lam: atan(t), |t| < kMaxTan
-    const double dl = atan_small(yp * rcp_nr(xp));
-    const double ta = fma(dl, R.inv_dphi(la), R.base(la));
-    const double tb = fma(dl, R.inv_dphi(la + 1), R.base(la + 1));
+    const double dl = atan_small(t);
+    // weight-grid cell coordinates on rings la, la + 1
+    const double ta = fma(dl, R.inv_dphi(la), R.baseo(la));
+    const double tb = fma(dl, R.inv_dphi(la + 1), R.baseo(la + 1));
     const int ia = (int)ta, ib = (int)tb;
+    // (a negative coordinate truncates towards zero, leaving a fraction
+    // <= 0, which the certification rejects)
     const double fa = ta - (double)ia, fb = tb - (double)ib;
-    ok &= (ta >= 0.0) & (tb >= 0.0) & (ia + 1 < R.width(la)) & (ib + 1 < R.width(la + 1));
+    const int2 owa = R.ow(la), owb = R.ow(la + 1);
+    ok &= ((unsigned)(ia - owa.x) < (unsigned)owa.y) & ((unsigned)(ib - owb.x) < (unsigned)owb.y);
     ok &= (fa > kCertSlot) & (fa < 1.0 - kCertSlot) & (fb > kCertSlot) & (fb < 1.0 - kCertSlot);
     // theta - theta_above = asin(sin(theta) z_a - z sin(theta_a))
     const double sq = sqrt_fast((1.0 - z) * (1.0 + z));
@@ -1069,12 +1125,11 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     float pp = fmaf(x2, 105.f / 3456.f, 15.f / 336.f);
     pp = fmaf(x2, pp, 3.f / 40.f);
     pp = fmaf(x2, pp, 1.f / 6.f);
-    const float bvf = (float)bv;
-    const float wb = fmaf(xf * x2, pp, xf) * (float)R.inv_dth(la) * bvf;
+    const float wb = fmaf(xf * x2, pp, xf) * R.inv_dth_f(la) * bvf;
     const float wa = bvf - wb;
     const float faf = (float)fa, fbf = (float)fb;
-    WCell* ca = W + R.off(la) + ia;
-    WCell* cb = W + R.off(la + 1) + ib;
+    WCell* ca = W + ia;
+    WCell* cb = W + ib;
     atomicAdd(ca, (WCell)(wa - wa * faf));
     atomicAdd(ca + 1, (WCell)(wa * faf));
     atomicAdd(cb, (WCell)(wb - wb * fbf));
@@ -1201,12 +1256,21 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
     // scatter into the same weight-grid cell (same-address LDS atomics
     // serialise)
     constexpr int kPer = kClaim / 64;
-    if (s0 + lane * kPer < S) {
-        nv0 = vec[3 * (s0 + lane * kPer)];
-        nv1 = vec[3 * (s0 + lane * kPer) + 1];
-        nv2 = vec[3 * (s0 + lane * kPer) + 2];
-        nbv = bvs[s0 + lane * kPer];
-    }
+    // beam pixel sb + kPer lane + it of a claim starting at sb (uniform),
+    // clamped to the last one so that the loads need no branch, through
+    // 32-bit byte offsets from the uniform bases (12 (S + kClaim) < 2^32)
+    const uint32_t lane12 = 12u * kPer * lane, lane4 = 4u * kPer * lane;
+    const uint32_t last12 = 12u * (uint32_t)(S - 1), last4 = 4u * (uint32_t)(S - 1);
+    auto fetch = [&](int sb, int it) {
+        const uint32_t o12 = min(12u * (uint32_t)(sb + it) + lane12, last12);
+        const uint32_t o4 = min(4u * (uint32_t)(sb + it) + lane4, last4);
+        const gptr<const float> vp = (gptr<const float>)((gptr<const char>)vec + o12);
+        nv0 = vp[0];
+        nv1 = vp[1];
+        nv2 = vp[2];
+        nbv = *(gptr<const float>)((gptr<const char>)bvs + o4);
+    };
+    if (S > 0) fetch(s0, 0);
     while (s0 < S) {
         const int s1 = claim();
         uint32_t slow = 0;
@@ -1214,20 +1278,17 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
         for (int it = 0; it < kPer; ++it) {
             const int s = s0 + lane * kPer + it;
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
-            const double bv = (double)nbv;
+            const float bvf = nbv;
             // fetch the next beam pixel early (the next chunk's first one
             // after the last iteration)
-            const int sn = it + 1 < kPer ? s + 1 : s1 + lane * kPer;
-            if (sn < S) {
-                nv0 = vec[3 * sn];
-                nv1 = vec[3 * sn + 1];
-                nv2 = vec[3 * sn + 2];
-                nbv = bvs[sn];
-            }
+            if (it + 1 < kPer)
+                fetch(s0, it + 1);
+            else
+                fetch(s1, 0);
             if (s < S) {
                 bool ok = false;
-                if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
-                if constexpr (MODE == 1) ok = fp_fast_nearest(m, nside, ir_lo, nr, R, W, v0, v1, v2, bv);
+                if constexpr (MODE == 0) ok = fp_fast_bilinear(m, nside, ir_lo, nr, R, W, v0, v1, v2, bvf);
+                if constexpr (MODE == 1) ok = fp_fast_nearest(m, nside, ir_lo, nr, R, W, v0, v1, v2, (double)bvf);
                 if (!ok) slow |= 1u << it;
             }
         }
@@ -1629,7 +1690,7 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
                     const float u0 = vec[3 * s], u1 = vec[3 * s + 1], u2 = vec[3 * s + 2], bvf = bvs[s];
                     ++n_fb;
                     if (!fp_fast_bilinear(m, nside, ir_lo, nr, R, W, (double)u0, (double)u1, (double)u2,
-                                          (double)bvf)) {
+                                          bvf)) {
                         const Acc4 d = fp_exact_pixel<0, QU, CS>(fp_globals(a), buf, u0, u1, u2, bvf);
 #pragma unroll
                         for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
@@ -1748,98 +1809,84 @@ constexpr float kSpinF32St2 = 3e-4f;
 
 // All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
 // gather per map pixel with nonzero weight; the cell is cleared for the next
-// sample.  Each wave takes 64 consecutive grid cells at a time (one per
-// lane), which span one or a few rings; the spin-2 terms that depend only on
-// the ring were computed with the layout.  Returns the per-thread sums
-// (outside-grid sums included).
-__device__ __forceinline__ int fp_ring_of(const FpRings& R, int nr, int r0, int j) {
-    // the ring of cell j >= off(r0): ring offsets are non-decreasing, so
-    // count the next few boundaries at or below j (independent LDS reads,
-    // uniform addresses); a chunk of 64 cells rarely spans more rings
-    int r = r0;
-#pragma unroll
-    for (int k = 1; k <= 4; ++k) r += (r0 + k < nr && R.off(r0 + k) <= j) ? 1 : 0;
-    if (r == r0 + 4)
-        while (r + 1 < nr && R.off(r + 1) <= j) ++r;
-    return r;
-}
-
-// All threads: phase 2 -- sweep the laid-out footprint once: spin-2 and one
-// gather per map pixel with nonzero weight; the cell is cleared for the next
-// sample.  A wave takes 64 consecutive grid cells at a time (one per lane,
-// spanning one or a few rings; the spin-2 terms that depend only on the
-// ring were computed with the layout), kP2Cells such chunks per round, so
-// that each lane has kP2Cells independent weight reads and gathers in
-// flight.  Returns the per-thread sums (outside-grid sums included).
+// sample.  Cell j of the grid lies on footprint ring rtab[j]; a wave takes
+// 64 consecutive cells at a time, kP2Cells such chunks per round, so that
+// each lane has kP2Cells independent weight reads and gathers in flight.
+// Returns the per-thread sums (outside-grid sums included).
 constexpr int kP2Cells = 5;
 
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
-                                          WCell* W, Acc4 acc) {
+                                          const unsigned char* rtab, WCell* W, Acc4 acc) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int nr = __builtin_amdgcn_readfirstlane(fr->nr);
     const int total = __builtin_amdgcn_readfirstlane(fr->total);
-    const int ir_lo = __builtin_amdgcn_readfirstlane(fr->ir_lo);
+    const int coinc = __builtin_amdgcn_readfirstlane(fr->coinc_cell);
     const FpSpin sp = fp_spin_of(fr);
-    int r0 = 0;  // ring of the current chunk's first cell (uniform, non-decreasing)
     for (int base = wave * 64; base < total; base += kFpThreads * kP2Cells) {
         double wsum[kP2Cells];
-        int rr[kP2Cells], pp[kP2Cells];
+        int rr[kP2Cells], kk[kP2Cells];
         float4 mv[kP2Cells];
 #pragma unroll
         for (int u = 0; u < kP2Cells; ++u) {
-            const int c0 = base + u * kFpThreads;
-            const int j = c0 + lane;
-            wsum[u] = j < total ? (double)W[j] : 0.0;
-            if (c0 < total) r0 = fp_ring_of(R, nr, r0, c0);  // uniform
-            rr[u] = r0;
+            const int j = base + u * kFpThreads + lane;
+            const bool in = j < total;
+            wsum[u] = in ? (double)W[j] : 0.0;
+            rr[u] = in ? (int)rtab[j] : 0;
         }
 #pragma unroll
         for (int u = 0; u < kP2Cells; ++u) {
             const int j = base + u * kFpThreads + lane;
-            pp[u] = 0;
+            kk[u] = 0;
             if (wsum[u] != 0.0) {
                 W[j] = (WCell)0;
-                const int r = fp_ring_of(R, nr, rr[u], j);
-                int pos = R.i0(r) + (j - R.off(r));
+                const int r = rr[u];
+                const int k = j - R.off(r);  // window slot
+                int pos = R.i0(r) + k;
                 const int n = R.n(r);
                 if (pos >= n) pos -= n;
-                rr[u] = r;
-                pp[u] = pos;
+                kk[u] = k;
                 mv[u] = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
             }
         }
 #pragma unroll
         for (int u = 0; u < kP2Cells; ++u) {
             if (wsum[u] == 0.0) continue;
-            const int r = rr[u], pos = pp[u];
+            const int j = base + u * kFpThreads + lane;
+            const int r = rr[u], k = kk[u];
             const double wsum_u = wsum[u];
             const float4 mv_u = mv[u];
             if constexpr (QU) {
                 double c2d, s2d;
-                // the half longitude gap relative to lam (a difference of angles)
-                const double hd = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
+                // half the longitude gap from the boresight to the pixel
+                const float gh = fmaf((float)k - R.kb(r), R.hdp(r), R.gb(r));
                 const float st2 = R.st2(r);
-                if (ir_lo + r == sp.coinc_ring && pos == sp.coinc_pos) {  // tod_bilinear.py:108-109
+                if (j == coinc) {  // tod_bilinear.py:108-109
                     c2d = 1.0;
                     s2d = 0.0;
-                } else if (st2 < kSpinF32St2 && fabs(hd) < 0.05) {
-                    // pixel within ~2 deg of the boresight: binary32
-                    float sb, cb, cf, sf;
-                    sincos_any_f((float)hd, sb, cb);
-                    const float sha = (float)sp.sha, cha = (float)sp.cha;
+                } else if (st2 < kSpinF32St2 && fabsf(gh) < 0.05f) {
+                    // pixel within ~2 deg of the boresight: binary32, sin and
+                    // cos of the half gap by Taylor series (|gh| < 0.05)
+                    const float g2 = gh * gh;
+                    const float sg = fmaf(gh * g2, fmaf(g2, 1.f / 120.f, -1.f / 6.f), gh);
+                    const float cg = fmaf(g2, fmaf(g2, fmaf(g2, -1.f / 720.f, 1.f / 24.f), -0.5f), 1.f);
                     Spin2RingF g;
                     g.st2 = st2;
                     g.S = R.sS(r);
                     g.s2dt = R.s2dt(r);
                     g.zsum = R.zsum(r);
                     g.SC4 = R.sc4(r);
-                    spin2_from_half_f(g, fmaf(sha, cb, -(cha * sb)), fmaf(cha, cb, sha * sb), cf, sf);
+                    float cf, sf;
+                    spin2_from_half_f(g, -sg, cg, cf, sf);
                     c2d = (double)cf;
                     s2d = (double)sf;
                 } else {
+                    int pos = R.i0(r) + k;
+                    const int n = R.n(r);
+                    if (pos >= n) pos -= n;
                     double sb, cb;
+                    // the half longitude gap relative to lam (a difference of angles)
+                    const double hd = (R.phi0(r) + (double)pos * R.dphi(r) - sp.lam) * 0.5;
                     sincos_any(hd, sb, cb);
                     const double sh = fma(sp.sha, cb, -(sp.cha * sb));
                     const double ch = fma(sp.cha, cb, sp.sha * sb);
@@ -1850,12 +1897,12 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
                 acc.v[0] = fma(wsum_u, fma(qv, c2d, uv * s2d), acc.v[0]);
                 acc.v[1] = fma(wsum_u, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
 #pragma unroll
-                for (int k = 2; k < CS; ++k)
-                    if (k < a.nslots) acc.v[k] = fma(wsum_u, (double)slot(mv_u, k), acc.v[k]);
+                for (int k2 = 2; k2 < CS; ++k2)
+                    if (k2 < a.nslots) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
             } else {
 #pragma unroll
-                for (int k = 0; k < CS; ++k)
-                    if (k < a.nslots) acc.v[k] = fma(wsum_u, (double)slot(mv_u, k), acc.v[k]);
+                for (int k2 = 0; k2 < CS; ++k2)
+                    if (k2 < a.nslots) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
             }
         }
     }
@@ -1897,10 +1944,9 @@ __device__ __forceinline__ void fp_flush(const TodArgs& a, const FpJob* J) {
     }
 }
 
-// Zero the weight grid and copy the reference-longitude table into LDS.
-__device__ __forceinline__ void fp_init(const double* lam_trig) {
+// Zero the weight grid.
+__device__ __forceinline__ void fp_init() {
     for (int j = threadIdx.x; j < kFpCap; j += kFpThreads) fp_W()[j] = (WCell)0;
-    for (int j = threadIdx.x; j < 2 * kLamN; j += kFpThreads) fp_trig()[j] = lam_trig[j];
     __syncthreads();
 }
 
@@ -1936,7 +1982,7 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     Acc4 acc = {{0.0, 0.0, 0.0, 0.0}};
     if (wave == 0 && have_next) {
         fp_set_job(an, bn, buf ^ 1);
-        fp_layout(an, bn, bfn, fp_trig(), fp_frame(buf ^ 1), fp_rings(buf ^ 1));
+        fp_layout(an, bn, bfn, fp_frame(buf ^ 1), fp_rings(buf ^ 1), fp_rtab(buf ^ 1));
     }
     // every wave, wave 0 once its layout is done: the job's claim counter
     // was zeroed when the job was set
@@ -1945,7 +1991,7 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     else
         acc = fp_phase1<MODE, QU, CS>(an, fp_job(buf), fr, R, buf, fp_W());
     __syncthreads();
-    acc = fp_phase2<QU, CS>(an, fr, R, fp_W(), acc);
+    acc = fp_phase2<QU, CS>(an, fr, R, fp_rtab(buf), fp_W(), acc);
     // mode 3: the next sample's values go into the value grid, which no
     // wave reads between the barrier above and the one below
     if constexpr (MODE == 3)
@@ -1969,7 +2015,7 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     const int64_t bbeg = (int64_t)xcd * chunk;
     const int64_t bend = (bbeg + chunk < a.B) ? bbeg + chunk : a.B;
     if (bbeg + kx >= bend) return;  // uniform over the block
-    fp_init(a.lam_trig);
+    fp_init();
     // footprint margin in pixel sizes: bilinear and nearest neighbour rings
     // lie within one ring spacing (<= 0.9) of the query, 4 x 4 corrector
     // neighbourhoods within ~3.6, bicubic candidates within ~4
@@ -1978,7 +2024,7 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     int64_t b = bbeg + kx;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
-        fp_layout(a, b, bf, fp_trig(), fp_frame(0), fp_rings(0));
+        fp_layout(a, b, bf, fp_frame(0), fp_rings(0), fp_rtab(0));
     }
     __syncthreads();
     CorrRegs C;
@@ -2138,14 +2184,14 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
     a.nslots = ba.nslots;
 #pragma unroll
     for (int k = 0; k < kMaxSlots; ++k) a.row[k] = ba.row[k];
-    fp_init(a.lam_trig);
+    fp_init();
     BatchCursor c = {0, 0, 0, -1, false, {0.0, 0.0, 1.0, 0.0, 1.0}};
     int64_t b = 0;
     const double margin_px = MODE == 2 ? 5.0 : 1.25;
     if (!batch_next(ba, c, a, b, margin_px)) return;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
-        fp_layout(a, b, c.bf, fp_trig(), fp_frame(0), fp_rings(0));
+        fp_layout(a, b, c.bf, fp_frame(0), fp_rings(0), fp_rtab(0));
     }
     __syncthreads();
     int buf = 0;
