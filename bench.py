@@ -32,7 +32,8 @@ sys.path.insert(0, REPO)
 METRIC = "beam-pix·samples/s at nside 2048; learned vs bilinear at bicubic-judged RMS err"
 UNIT = "beam-pix·samples/s"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_BPS = {"bilinear": 48, "nearest": 12}  # neighbours x 3 fields x float32
+# neighbours x 3 fields x float32 (learned: the corrector's 4 x 4 neighbourhood)
+BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 192}
 
 
 def log(msg):
@@ -123,7 +124,9 @@ def main():
     ap.add_argument("--fwhm", type=float, default=30.0)
     ap.add_argument("--samples", type=int, default=1728000)
     ap.add_argument("--fs", type=float, default=20.0)
-    ap.add_argument("--mode", choices=["bilinear", "nearest"], default="bilinear")
+    ap.add_argument("--mode", choices=["bilinear", "nearest", "learned"], default="bilinear",
+                    help="learned: the fused corrector kernel with seeded random parameters "
+                         "(its work does not depend on their values), bound 4")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
@@ -158,7 +161,15 @@ def main():
     dbv = to_dev(bv, f32)
     dparams = [to_dev(p, f32) for p in params]
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
-    fn = kernels._gather_accum_fused_jit if args.mode == "bilinear" else kernels._gather_accum_nearest_jit
+    if args.mode == "learned":
+        from tod_generation_mapbased_beam_amd import corrector
+
+        corr = torch.as_tensor(corrector.init_params(0) * 2.0, device=dev)
+
+        def fn(*a, variant=0):
+            corrector.gather_accum_learned(*a, params=corr, bound=4.0)
+    else:
+        fn = kernels._gather_accum_fused_jit if args.mode == "bilinear" else kernels._gather_accum_nearest_jit
 
     def step(ev0=None, ev1=None):
         tod.zero_()
@@ -196,7 +207,7 @@ def main():
     finite = bool(torch.isfinite(tod).all())
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.mode != "learned":
         mp_host = mp.cpu().numpy()
         cpu = cpu_baseline(vec, bv, params, mp_host, args.nside, B, S, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.3e} bp.s/s on {cpu['cores']} cores")
