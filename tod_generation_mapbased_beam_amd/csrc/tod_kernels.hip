@@ -1188,12 +1188,13 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
         const double fr = tr - (double)it;
         ok = (fr > kCertPolar) & (fr < 1.0 - kCertPolar) & (az < 1.0);
     }
-    ok &= fabs(az - kTwoThirds) > 1e-12;
+    // (the belt/cap choice and the ring range as in fp_fast_bilinear)
     int la = ira - ir_lo;
-    ok &= (ira >= 1) & (ira <= 4 * nside - 2) & (la >= 0) & (la + 1 < nr);
-    ok &= (xp > 0.0) & (fabs(yp) < kMaxTan * xp);
+    ok &= (unsigned)la < (unsigned)(nr > 1 ? nr - 1 : 0);
+    const double t = yp * rcp_nr1(xp);
+    ok &= (xp > 0.0) & (fabs(t) < kMaxTan);
     la = ok ? la : 0;
-    const double dl = atan_small(yp * rcp_nr(xp));
+    const double dl = atan_small(t);
     const double sth = sqrt_fast(fmax(fma(xp, xp, yp * yp), 1e-300));
     double best = -2.0, second = -2.0;
     int cell = 0;
@@ -1206,8 +1207,8 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
         const double base = R.base(r) + half;
         const double tc = fma(dl, R.inv_dphi(r), base);
         const int j = (int)tc;
-        const double f = tc - (double)j;
-        ok &= (tc >= 0.0) & (j + 1 < R.width(r)) & (f > kCertSlot) & (f < 1.0 - kCertSlot);
+        const double f = tc - (double)j;  // <= 0 for tc < 0: rejected below
+        ok &= (j + 1 < R.width(r)) & (f > kCertSlot) & (f < 1.0 - kCertSlot);
         const double ss = sth * R.sth(r), zz = z * R.z(r), dphi = R.dphi(r);
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
@@ -1232,6 +1233,7 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
 // kClaim beam pixels at a time from the sample's counter; wave 0 joins once
 // it has laid out the next sample.
 constexpr int kClaim = 256;  // beam pixels per claim (4 per lane)
+constexpr int kClaimSub = 2;  // consecutive beam pixels per lane in a claim's block
 
 template <int MODE, bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFrame* fr,
@@ -1258,19 +1260,24 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
     };
     int s0 = claim();
     float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
-    // lane l sweeps beam pixels s0 + kPer l .. + kPer - 1 of a claim: in one
-    // iteration the wave's lanes sit kPer pixels apart, so fewer of them
-    // scatter into the same weight-grid cell (same-address LDS atomics
-    // serialise)
+    // a claim is kPer / kSub blocks of 64 kSub beam pixels; in block k lane l
+    // takes pixels 64 kSub k + kSub l .. + kSub - 1, so in one iteration the
+    // wave's lanes sit kSub pixels apart: few of them scatter into the same
+    // weight-grid cell (same-address LDS atomics serialise) or, at the ~1
+    // cell per 2 pixels of the beams here, into the same LDS bank
     constexpr int kPer = kClaim / 64;
-    // beam pixel sb + kPer lane + it of a claim starting at sb (uniform),
-    // clamped to the last one so that the loads need no branch, through
-    // 32-bit byte offsets from the uniform bases (12 (S + kClaim) < 2^32)
-    const uint32_t lane12 = 12u * kPer * lane, lane4 = 4u * kPer * lane;
+    constexpr int kSub = kClaimSub;
+    static_assert(kPer % kSub == 0, "whole blocks per claim");
+    // claim-relative index of a lane's it-th beam pixel, minus kSub lane
+    auto in_claim = [](int it) { return (it / kSub) * 64 * kSub + it % kSub; };
+    // beam pixel sb + in_claim(it) + kSub lane of a claim starting at sb
+    // (uniform), clamped to the last one so that the loads need no branch,
+    // through 32-bit byte offsets from the uniform bases (12 (S + kClaim) < 2^32)
+    const uint32_t lane12 = 12u * kSub * lane, lane4 = 4u * kSub * lane;
     const uint32_t last12 = 12u * (uint32_t)(S - 1), last4 = 4u * (uint32_t)(S - 1);
     auto fetch = [&](int sb, int it) {
-        const uint32_t o12 = min(12u * (uint32_t)(sb + it) + lane12, last12);
-        const uint32_t o4 = min(4u * (uint32_t)(sb + it) + lane4, last4);
+        const uint32_t o12 = min(12u * (uint32_t)(sb + in_claim(it)) + lane12, last12);
+        const uint32_t o4 = min(4u * (uint32_t)(sb + in_claim(it)) + lane4, last4);
         const gptr<const float> vp = (gptr<const float>)((gptr<const char>)vec + o12);
         nv0 = vp[0];
         nv1 = vp[1];
@@ -1283,7 +1290,7 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
         uint32_t slow = 0;
 #pragma unroll
         for (int it = 0; it < kPer; ++it) {
-            const int s = s0 + lane * kPer + it;
+            const int s = s0 + in_claim(it) + lane * kSub;
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
             const float bvf = nbv;
             // fetch the next beam pixel early (the next chunk's first one
@@ -1303,7 +1310,7 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
         while (slow) {
             const int it = __builtin_ctz(slow);
             slow &= slow - 1;
-            const int ss = s0 + lane * kPer + it;
+            const int ss = s0 + in_claim(it) + lane * kSub;
             const Acc4 d = fp_exact_pixel<MODE, QU, CS>(fp_globals(a), buf, vec[3 * ss],
                                                         vec[3 * ss + 1], vec[3 * ss + 2], bvs[ss]);
 #pragma unroll
