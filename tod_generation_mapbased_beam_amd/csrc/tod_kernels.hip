@@ -468,7 +468,11 @@ constexpr int kOffFrame = kOffRings + 2 * kRingBuf;
 constexpr int kOffRed = kOffFrame + 2 * (int)sizeof(FpFrame);
 constexpr int kOffScratch = kOffRed + kFpWaves * kMaxSlots * 8;
 constexpr int kOffJob = kOffScratch + (kFpWaves + 2) * 8;
-constexpr int kOffW = kOffJob + 2 * 64;
+// the batched kernel's current item (TodArgs) and beam frame, kept in LDS
+// rather than in registers across a step
+constexpr int kOffItem = kOffJob + 2 * 64;
+constexpr int kItemBytes = 256;
+constexpr int kOffW = kOffItem + kItemBytes;
 // weight-grid cell type (beam value x interpolation weight sums)
 typedef double WCell;
 // ring of each weight-grid cell (footprint-relative, < kFpMaxRings <= 256),
@@ -1234,6 +1238,18 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
 // it has laid out the next sample.
 constexpr int kClaim = 256;  // beam pixels per claim (4 per lane)
 constexpr int kClaimSub = 2;  // consecutive beam pixels per lane in a claim's block
+static_assert((kClaim / 64) % kClaimSub == 0, "whole blocks per claim");
+
+// A claim is kClaim / (64 kClaimSub) blocks of 64 kClaimSub beam pixels; in
+// block k lane l takes pixels 64 kClaimSub k + kClaimSub l .. + kClaimSub - 1,
+// so in one iteration the wave's lanes sit kClaimSub pixels apart: few of
+// them scatter into the same weight-grid cell (same-address LDS atomics
+// serialise) or, at the ~1 cell per 2 pixels of the beams here, into the
+// same LDS bank.  Claim-relative index of a lane's it-th beam pixel, minus
+// kClaimSub lane:
+__device__ __forceinline__ constexpr int claim_offset(int it) {
+    return (it / kClaimSub) * 64 * kClaimSub + it % kClaimSub;
+}
 
 template <int MODE, bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFrame* fr,
@@ -1260,16 +1276,9 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
     };
     int s0 = claim();
     float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
-    // a claim is kPer / kSub blocks of 64 kSub beam pixels; in block k lane l
-    // takes pixels 64 kSub k + kSub l .. + kSub - 1, so in one iteration the
-    // wave's lanes sit kSub pixels apart: few of them scatter into the same
-    // weight-grid cell (same-address LDS atomics serialise) or, at the ~1
-    // cell per 2 pixels of the beams here, into the same LDS bank
     constexpr int kPer = kClaim / 64;
     constexpr int kSub = kClaimSub;
-    static_assert(kPer % kSub == 0, "whole blocks per claim");
-    // claim-relative index of a lane's it-th beam pixel, minus kSub lane
-    auto in_claim = [](int it) { return (it / kSub) * 64 * kSub + it % kSub; };
+    auto in_claim = [](int it) { return claim_offset(it); };
     // beam pixel sb + in_claim(it) + kSub lane of a claim starting at sb
     // (uniform), clamped to the last one so that the loads need no branch,
     // through 32-bit byte offsets from the uniform bases (12 (S + kClaim) < 2^32)
@@ -1694,12 +1703,12 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
             s0 = (int64_t)__builtin_amdgcn_readfirstlane(__shfl(c, 0, 64)) * kClaim;
             it = 0;
         }
-        if (s0 + it >= S) break;  // no lane has a beam pixel left (uniform)
+        if (s0 + claim_offset(it) >= S) break;  // no lane has a beam pixel left (uniform)
         if (s0 >= K) {
             // the whole claim lies beyond the corrected beam pixels: the
             // plain bilinear path, no MLP (uniform branch)
             for (; it < kPer; ++it) {
-                const int64_t s = s0 + (int64_t)lane * kPer + it;
+                const int64_t s = s0 + (int64_t)lane * kClaimSub + claim_offset(it);
                 if (s < S) {
                     const float u0 = vec[3 * s], u1 = vec[3 * s + 1], u2 = vec[3 * s + 2], bvf = bvs[s];
                     ++n_fb;
@@ -1713,8 +1722,8 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
             }
             continue;
         }
-        // lanes kPer beam pixels apart (fewer same-cell LDS atomics)
-        const int64_t s = s0 + (int64_t)lane * kPer + it;
+        // lanes kClaimSub beam pixels apart (claim_offset)
+        const int64_t s = s0 + (int64_t)lane * kClaimSub + claim_offset(it);
         ++it;
         const bool live = s < S;
         float u0 = 0.f, u1 = 0.f, u2 = 0.f, bvf = 0.f;
@@ -2120,19 +2129,20 @@ __device__ __forceinline__ bool finite_sample(const TodArgs& a, int64_t b) {
 // chunk lengths balance themselves.  Every workgroup exits when all eight
 // queues are drained.  A sample with a non-finite pointing operand is
 // skipped and flags its item's status word.
+static_assert(((sizeof(TodArgs) + 15) & ~15) + sizeof(BeamFrame) <= kItemBytes, "item slot");
+
 struct BatchCursor {
     int att;          // queues tried so far
     int64_t g, g1;    // current claim [g, g1)
     int cur;          // item of `a`
     bool have_frame;
-    BeamFrame bf;
 };
 
 // All threads of the block (uniform): advance to the next sample to process;
-// loads its item into `a`, its beam frame into c.bf.  Returns false when
+// loads its item into `a`, its beam frame into `bf`.  Returns false when
 // every queue is drained.
 __device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, TodArgs& a,
-                                           int64_t& b, double margin_px) {
+                                           BeamFrame& bf, int64_t& b, double margin_px) {
     const int tid = threadIdx.x;
     const int xcd = blockIdx.x & 7;
     const int64_t qlen = (ba.total + 7) / 8;
@@ -2179,7 +2189,8 @@ __device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, 
             continue;
         }
         if (!c.have_frame) {  // beam frame from the item's first finite sample seen here
-            c.bf = fp_beam_frame(beam_src(a), b, fp_scratch(), margin_px);
+            const BeamFrame f = fp_beam_frame(beam_src(a), b, fp_scratch(), margin_px);
+            bf = f;
             c.have_frame = true;
         }
         return true;
@@ -2189,7 +2200,9 @@ __device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, 
 template <int MODE, bool QU, int CS, int MINW>
 __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchArgs ba,
                                                                    int unused) {
-    TodArgs a;
+    // item and beam frame in LDS (every thread writes the same values)
+    TodArgs& a = lds_at<TodArgs>(kOffItem);
+    BeamFrame& bf = lds_at<BeamFrame>(kOffItem + (((int)sizeof(TodArgs) + 15) & ~15));
     a.map = ba.map;
     a.rings = ba.rings;
     a.lam_trig = ba.lam_trig;
@@ -2199,20 +2212,20 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp_batch(const BatchAr
 #pragma unroll
     for (int k = 0; k < kMaxSlots; ++k) a.row[k] = ba.row[k];
     fp_init();
-    BatchCursor c = {0, 0, 0, -1, false, {0.0, 0.0, 1.0, 0.0, 1.0}};
+    BatchCursor c = {0, 0, 0, -1, false};
     int64_t b = 0;
     const double margin_px = MODE == 2 ? 5.0 : 1.25;
-    if (!batch_next(ba, c, a, b, margin_px)) return;
+    if (!batch_next(ba, c, a, bf, b, margin_px)) return;
     if ((threadIdx.x >> 6) == 0) {
         fp_set_job(a, b, 0);
-        fp_layout(a, b, c.bf, fp_frame(0), fp_rings(0), fp_rtab(0));
+        fp_layout(a, b, bf, fp_frame(0), fp_rings(0), fp_rtab(0));
     }
     __syncthreads();
     int buf = 0;
     for (;;) {
         int64_t bn = 0;
-        const bool have_next = batch_next(ba, c, a, bn, margin_px);
-        fp_step<MODE, QU, CS>(a, have_next, bn, c.bf, buf);
+        const bool have_next = batch_next(ba, c, a, bf, bn, margin_px);
+        fp_step<MODE, QU, CS>(a, have_next, bn, bf, buf);
         if (!have_next) break;
         buf ^= 1;
     }
