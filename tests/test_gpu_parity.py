@@ -164,6 +164,32 @@ def test_q_one_at_boresight(K):
     np.testing.assert_allclose(t[2], 0.0, atol=1e-3)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
+def test_spin2_guard_at_coincident_pixel(K, mode, variant):
+    """Boresight exactly on a map pixel centre: the reference returns
+    (cos 2d, sin 2d) = (1, 0) for that pixel (tod_bilinear.py:108-109).  At
+    nside 1 the equator ring's pixel 5 sits at phi = pi/2, z = 0, which the
+    float32 boresight (0, 1, 0) hits bit for bit; the beam spreads around it
+    so that the coincident pixel and its neighbours all carry weight."""
+    nside = 1
+    rng = np.random.default_rng(61)
+    ax = np.array([[0.0, 1.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.6, 0.8]], np.float32)
+    params = recenter_params(ax)
+    S = 40
+    v = rng.standard_normal((S, 3)) * np.array([0.3, 0.3, 0.0]) + np.array([0.0, 0.0, 1.0])
+    v[0] = [0.0, 0.0, 1.0]
+    vec = (v / np.linalg.norm(v, axis=-1, keepdims=True)).astype(np.float32)
+    mp = rng.uniform(-1.0, 1.0, (3, 12)).astype(np.float32)
+    bv = rng.uniform(0.1, 1.0, S).astype(np.float32)
+    bv /= bv.sum()
+    B = ax.shape[0]
+    g = _gpu_tod(K, mode, vec, params, nside, mp, bv, B, S, 1, 2, variant=variant)
+    r = _cpu_tod(mode, vec, params, nside, mp, bv, B, S, 1, 2)
+    assert np.isfinite(g).all()  # no 0/0 from the coincident pixel's spin-2 terms
+    assert rel_err(g, r) <= TOL
+
+
 def test_beam_tod_batch_dropin():
     """The reference's dispatcher contract end to end (numpy in, dict of
     float32 out) against the oracle."""
