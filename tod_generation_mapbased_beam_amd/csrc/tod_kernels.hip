@@ -833,6 +833,18 @@ __device__ __noinline__ BeamFrame fp_beam_frame(BeamSrc a, int64_t b0, double* r
 // slots floor(L) - 1 .. at least floor(R) + 2 (the right neighbour of the
 // last slot and one more); the reference longitude lam and everything the
 // index path derives from it are binary64.
+// Ring table entries [k0, k1) = r: single bytes up to a 4-byte boundary and
+// after the last one, whole words between (the table base is 4-byte
+// aligned; a neighbouring ring's bytes in a shared word are written by
+// their own byte stores).
+__device__ __forceinline__ void fill_ring_cells(unsigned char* rtab, int k0, int k1, int r) {
+    const unsigned rep4 = (unsigned)r * 0x01010101u;
+    int k = k0;
+    for (; k < k1 && (k & 3) != 0; ++k) rtab[k] = (unsigned char)r;
+    for (; k + 4 <= k1; k += 4) *reinterpret_cast<unsigned*>(rtab + k) = rep4;
+    for (; k < k1; ++k) rtab[k] = (unsigned char)r;
+}
+
 __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const BeamFrame& bf,
                                           FpFrame* fr, FpRings R, unsigned char* rtab) {
     const int lane = threadIdx.x & 63;
@@ -971,7 +983,7 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
             R.baseo(r) = base + (double)off;
             R.ow(r) = make_int2(off, fits && width > 1 ? width - 1 : 0);
             if (fits)
-                for (int k = 0; k < width; ++k) rtab[off + k] = (unsigned char)r;
+                fill_ring_cells(rtab, off, off + width, r);
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) end = max(end, __shfl_xor(end, d, 64));
