@@ -918,7 +918,9 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         int width = 0, i0 = 0;
         double base = 0.0;
         if (r < nr) {
-            const RingEntry& g = a.rings[ir_lo + r];
+            // (a copy: all of the row's loads are issued before the LDS
+            // stores below, which the compiler would not move them past)
+            const RingEntry g = a.rings[ir_lo + r];
             const float nf = (float)g.n;
             const double tc = (lam - g.phi0) * g.inv_dphi;  // lam's slot coordinate
             const float den = fstf * (float)g.sth;
@@ -1026,15 +1028,25 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
         // and sin(theta) equal the boresight's bit for bit (e.g. z = 0).
         int cr = -1, cp = -1;
         const int ic = ring_above(nside, z_pts);
-        for (int ir = ic - 1; ir <= ic + 1; ++ir) {
+        const RingEntry* rings = a.rings;
+        double cz[3], cs[3];  // the candidate rings' z and sin(theta), loaded together
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int ir = min(max(ic - 1 + k, 1), 4 * nside - 1);
+            cz[k] = rings[ir].z;
+            cs[k] = rings[ir].sth;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int ir = ic - 1 + k;
             if (ir < 1 || ir > 4 * nside - 1) continue;
-            const RingEntry& g = a.rings[ir];
-            if (g.z == z_pts && g.sth == sth_pts) {
+            if (cz[k] == z_pts && cs[k] == sth_pts) {
+                const RingEntry& g = a.rings[ir];
                 const double ph = boresight_phi(q.bx, q.by);
-                const double k = rint((ph - g.phi0) / g.dphi);
-                if (k >= 0.0 && k < (double)g.n && g.phi0 + k * g.dphi == ph) {
+                const double kp = rint((ph - g.phi0) / g.dphi);
+                if (kp >= 0.0 && kp < (double)g.n && g.phi0 + kp * g.dphi == ph) {
                     cr = ir;
-                    cp = (int)k;
+                    cp = (int)kp;
                 }
             }
         }
