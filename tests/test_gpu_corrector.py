@@ -72,9 +72,16 @@ def test_loss_and_gradient_match_oracle(torch):
     d = np.abs(pred.cpu().numpy() - po)
     assert np.quantile(d, 0.99) <= 2e-4 and d.max() <= 1e-2
     gg = g.cpu().numpy().astype(np.float64)
-    for name_off, n in ((0, 64 * 21), (1344, 64), (1408, 4096), (5504, 64), (5568, 1024), (6592, 16)):
-        a, b = gg[name_off:name_off + n], go[name_off:name_off + n]
-        assert np.max(np.abs(a - b)) <= 2e-3 * np.max(np.abs(b)) + 1e-9, name_off
+    from tod_generation_mapbased_beam_amd.corrector import N_H, N_IN, N_OUT
+
+    # per parameter block of the flat layout W1 b1 W2 b2 W3 b3
+    sizes = (N_H * N_IN, N_H, N_H * N_H, N_H, N_OUT * N_H, N_OUT)
+    assert sum(sizes) == gg.size == go.size
+    off = 0
+    for n in sizes:
+        a, b = gg[off:off + n], go[off:off + n]
+        assert np.max(np.abs(a - b)) <= 2e-3 * np.max(np.abs(b)) + 1e-9, off
+        off += n
 
 
 def test_adam_step_matches_formula(torch):

@@ -484,8 +484,8 @@ constexpr int kFpSmemBytes = kOffRtab + 2 * kFpCap;
 constexpr int kOffIv = kFpSmemBytes;
 constexpr int kOffStage = kOffIv + kFpCap * 4;
 constexpr int kStageBytes = 64 * 64;
-constexpr int kOffBias = kOffStage + kFpWaves * kStageBytes;  // b2 (64), b3 (16) as float
-constexpr int kFpSmemBytesCorr = kOffBias + 80 * 4;
+constexpr int kOffBias = kOffStage + kFpWaves * kStageBytes;  // b2 (kH), b3 (kOut) as float
+constexpr int kFpSmemBytesCorr = kOffBias + (todg_corr::kH + todg_corr::kOut) * 4;
 static_assert(kOffStage % 16 == 0 && kOffBias % 16 == 0, "16-byte LDS vectors");
 static_assert(kFpSmemBytesCorr <= 160 * 1024, "one workgroup per CU");
 static_assert(kFpSmemBytes <= 80 * 1024, "two workgroups per CU");
@@ -1368,10 +1368,14 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+constexpr int kCorrHT = todg_corr::kH / 16;  // hidden features in output tiles of 16
+constexpr int kCorrHB = todg_corr::kH / 32;  // hidden features in k-blocks of 32
+static_assert(todg_corr::kH % 32 == 0 && todg_corr::kOut == 16, "corrector tiling");
+
 struct CorrRegs {
-    bf16x8 w1[4];
-    bf16x8 w2[4][2];
-    bf16x8 w3[2];
+    bf16x8 w1[kCorrHT];
+    bf16x8 w2[kCorrHT][kCorrHB];
+    bf16x8 w3[kCorrHB];
 };
 
 __device__ __forceinline__ int corr_perm(int kb, int g, int j) {
@@ -1382,7 +1386,7 @@ __device__ __forceinline__ void corr_load(const float* __restrict__ P, CorrRegs&
     using namespace todg_corr;
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
 #pragma unroll
-    for (int ft = 0; ft < 4; ++ft) {
+    for (int ft = 0; ft < kCorrHT; ++ft) {
         const int o = 16 * ft + r;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -1391,12 +1395,12 @@ __device__ __forceinline__ void corr_load(const float* __restrict__ P, CorrRegs&
             c.w1[ft][j] = (__bf16)w;
         }
 #pragma unroll
-        for (int kb = 0; kb < 2; ++kb)
+        for (int kb = 0; kb < kCorrHB; ++kb)
 #pragma unroll
             for (int j = 0; j < 8; ++j) c.w2[ft][kb][j] = (__bf16)P[kOffW2 + o * kH + corr_perm(kb, g, j)];
     }
 #pragma unroll
-    for (int kb = 0; kb < 2; ++kb)
+    for (int kb = 0; kb < kCorrHB; ++kb)
 #pragma unroll
         for (int j = 0; j < 8; ++j) c.w3[kb][j] = (__bf16)P[kOffW3 + r * kH + corr_perm(kb, g, j)];
     // b2, b3 (FP32) go to LDS: each lane reads the four of its output rows
@@ -1420,26 +1424,33 @@ __device__ __forceinline__ bf16x8 corr_act(const f32x4& lo, const f32x4& hi) {
 // on return o[] holds the lane's 16 outputs (the tile is reused for them).
 __device__ __forceinline__ void corr_mlp(const CorrRegs& c, unsigned char* tile, float o[16]) {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
-    const f32x4* bias = reinterpret_cast<const f32x4*>(&lds_at<float>(kOffBias));  // [ft * 4 + g], b3 at [16 + g]
+    // b2 as [ft * 4 + g], b3 at [kH / 4 + g]
+    const f32x4* bias = reinterpret_cast<const f32x4*>(&lds_at<float>(kOffBias));
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         unsigned char* row = tile + (16 * t + r) * 64 + 16 * g;
         const bf16x8 x = *reinterpret_cast<const bf16x8*>(row);
         const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-        f32x4 h[4];
+        f32x4 h[kCorrHT];
+        bf16x8 a[kCorrHB];
 #pragma unroll
-        for (int ft = 0; ft < 4; ++ft) h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w1[ft], x, zero, 0, 0, 0);
-        const bf16x8 a0 = corr_act(h[0], h[1]);
-        const bf16x8 a1 = corr_act(h[2], h[3]);
+        for (int ft = 0; ft < kCorrHT; ++ft)
+            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w1[ft], x, zero, 0, 0, 0);
 #pragma unroll
-        for (int ft = 0; ft < 4; ++ft) {
-            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][0], a0, bias[ft * 4 + g], 0, 0, 0);
-            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][1], a1, h[ft], 0, 0, 0);
+        for (int kb = 0; kb < kCorrHB; ++kb) a[kb] = corr_act(h[2 * kb], h[2 * kb + 1]);
+#pragma unroll
+        for (int ft = 0; ft < kCorrHT; ++ft) {
+            h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][0], a[0], bias[ft * 4 + g], 0, 0, 0);
+#pragma unroll
+            for (int kb = 1; kb < kCorrHB; ++kb)
+                h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w2[ft][kb], a[kb], h[ft], 0, 0, 0);
         }
-        const bf16x8 e0 = corr_act(h[0], h[1]);
-        const bf16x8 e1 = corr_act(h[2], h[3]);
-        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[0], e0, bias[16 + g], 0, 0, 0);
-        out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[1], e1, out, 0, 0, 0);
+#pragma unroll
+        for (int kb = 0; kb < kCorrHB; ++kb) a[kb] = corr_act(h[2 * kb], h[2 * kb + 1]);
+        f32x4 out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[0], a[0], bias[todg_corr::kH / 4 + g], 0, 0, 0);
+#pragma unroll
+        for (int kb = 1; kb < kCorrHB; ++kb)
+            out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(c.w3[kb], a[kb], out, 0, 0, 0);
         *reinterpret_cast<f32x4*>(row) = out;  // row 16t + r, outputs 4g .. 4g + 3
     }
     const f32x4* mine = reinterpret_cast<const f32x4*>(tile + lane * 64);
