@@ -1192,7 +1192,15 @@ __device__ __forceinline__ double cos_tiny(double x) {
 // between these cos-distances and the reference's, so it is the reference's
 // winner (ties, where the reference keeps the first candidate, fall to the
 // exact path).  The winner's cell gets the beam value.
+//
+// The two candidates of one ring share sin(theta_r) and z_r, so their
+// cos-distances ss cos(g) + zz order as |g| does: only each ring's nearer
+// candidate needs its cosine, and its lead over the farther one,
+// ss (cos g_w - cos g_l) = ss (g_l^2 - g_w^2) / 2 (1 - (g_l^2 + g_w^2) / 12
+// + ...), is at least kHalfLow ss (g_l^2 - g_w^2) for |g| <= 0.05 (the
+// series alternates with falling terms; the bracket is >= 1 - 0.005 / 12).
 constexpr double kCertNearest = 1e-12;
+constexpr double kHalfLow = 0.4995;
 
 __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, int nside, int ir_lo,
                                                 int nr, const FpRings& R, WCell* W, double u0,
@@ -1224,8 +1232,8 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
     la = ok ? la : 0;
     const double dl = atan_small(t);
     const double sth = sqrt_fast(fmax(fma(xp, xp, yp * yp), 1e-300));
-    double best = -2.0, second = -2.0;
-    int cell = 0;
+    double d[2], lead[2];
+    int c[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int r = la + k;
@@ -1238,19 +1246,22 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
         const double f = tc - (double)j;  // <= 0 for tc < 0: rejected below
         ok &= (j + 1 < R.width(r)) & (f > kCertSlot) & (f < 1.0 - kCertSlot);
         const double ss = sth * R.sth(r), zz = z * R.z(r), dphi = R.dphi(r);
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            // phi_w - phi_c = dl + dphi (base - half - (j + q)) in window-relative slots
-            const double g = fma(dphi, base - half - (double)(j + q), dl);
-            ok &= fabs(g) <= 0.05;
-            const double d = fma(ss, cos_tiny(g), zz);
-            const bool win = d > best;
-            second = win ? best : fmax(second, d);
-            cell = win ? R.off(r) + j + q : cell;
-            best = win ? d : best;
-        }
+        // phi_w - phi_c = dl + dphi (base - half - (j + q)) in window-relative slots
+        const double g0 = fma(dphi, base - half - (double)j, dl);
+        const double g1 = fma(dphi, base - half - (double)(j + 1), dl);
+        const bool far0 = fabs(g1) < fabs(g0);
+        const double gw = far0 ? g1 : g0, gl = far0 ? g0 : g1;
+        ok &= fabs(gl) <= 0.05;
+        d[k] = fma(ss, cos_tiny(gw), zz);
+        lead[k] = kHalfLow * ss * fma(gl, gl, -gw * gw);
+        c[k] = R.off(r) + j + (far0 ? 1 : 0);
     }
-    ok &= best - second > kCertNearest;
+    // best of the two rings' nearer candidates; the runner-up is the other
+    // ring's nearer candidate or the best ring's farther one
+    const bool w1 = d[1] > d[0];
+    const double best = w1 ? d[1] : d[0], other = w1 ? d[0] : d[1];
+    const int cell = w1 ? c[1] : c[0];
+    ok &= (best - other > kCertNearest) & ((w1 ? lead[1] : lead[0]) > kCertNearest);
     if (!ok) return false;
     atomicAdd(W + cell, (WCell)bv);
     return true;
