@@ -163,16 +163,16 @@ int todg_beam_bell(const double *x, const double *w, int64_t N, int lmax, double
 /* ------------------------------------------------------------------------
  * Learned interpolation corrector (no counterpart in the reference: the
  * build's corrector of bilinear interpolation toward the bicubic judge).
- * An MLP 21 -> 32 (GELU) -> 32 (GELU) -> 16 with FP32 master parameters in
- * one flat array of TODG_CORR_NPARAMS floats: W1 (32 x 21), b1, W2 (32 x 32),
- * b2, W3 (16 x 32), b3 (row-major, output x input).  Inputs: the query's
+ * An MLP 21 -> 64 (GELU) -> 64 (GELU) -> 16 with FP32 master parameters in
+ * one flat array of TODG_CORR_NPARAMS floats: W1 (64 x 21), b1, W2 (64 x 64),
+ * b2, W3 (16 x 64), b3 (row-major, output x input).  Inputs: the query's
  * sub-pixel coordinates in its 4 x 4 ring neighbourhood (fa, v, fb, fm, fp,
  * each minus 1/2: slot fractions in rings ira, ira+1, ira-1, ira+2 and the
  * bilinear ring weight v) and the 16 neighbourhood values normalised by
  * their mean and RMS; outputs: 16 weight corrections (their mean is
  * removed).  Matrix products run on BF16 MFMA with FP32 accumulation.  All
  * pointers are device pointers. */
-#define TODG_CORR_NPARAMS 2288
+#define TODG_CORR_NPARAMS 6608
 
 /* Features of N query directions (z, phi: float64) on one RING-ordered map
  * field (npix float32): X (N x 21: fa, v, fb, fm, fp minus 1/2, the 4 x 4

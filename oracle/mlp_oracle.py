@@ -28,7 +28,7 @@ import math
 import numpy as np
 
 GEO = 5
-N_IN, N_H, N_OUT = GEO + 16, 32, 16
+N_IN, N_H, N_OUT = GEO + 16, 64, 16
 SIZES = [("W1", (N_H, N_IN)), ("b1", (N_H,)), ("W2", (N_H, N_H)), ("b2", (N_H,)),
          ("W3", (N_OUT, N_H)), ("b3", (N_OUT,))]
 N_PARAMS = sum(int(np.prod(s)) for _, s in SIZES)

@@ -1,7 +1,7 @@
 """Learned interpolation corrector: training data, training loop and point
 prediction on the GPU (kernels in csrc/corrector.hip).
 
-The corrector is an MLP 21 -> 32 -> 32 -> 16 (GELU) that looks at the
+The corrector is an MLP 21 -> 64 -> 64 -> 16 (GELU) that looks at the
 sub-pixel position of a direction in the 4 x 4 ring neighbourhood of its
 bilinear cell (the slot fractions in the four rings and the bilinear ring
 weight) and at the neighbourhood's 16 map values (normalised by their mean
@@ -27,9 +27,9 @@ import torch
 from . import _lib, inputs, kernels
 from .device import device, stream_handle, to_dev
 
-N_PARAMS = 2288
+N_PARAMS = 6608
 GEO = 5
-N_IN, N_H, N_OUT = GEO + 16, 32, 16
+N_IN, N_H, N_OUT = GEO + 16, 64, 16
 
 
 def init_params(seed: int) -> np.ndarray:

@@ -10,8 +10,8 @@
 //
 //   x  = [fa - 1/2, v - 1/2, fb - 1/2, fm - 1/2, fp - 1/2,
 //         (m_k - mu) / sigma for k < 16]                          (21)
-//   h1 = gelu(W1 x + b1)        (32)    gelu: see below
-//   h2 = gelu(W2 h1 + b2)       (32)
+//   h1 = gelu(W1 x + b1)        (64)    gelu: see below
+//   h2 = gelu(W2 h1 + b2)       (64)
 //   o  = W3 h2 + b3             (16)
 //   w  = w_bilinear + o - mean(o)
 //   prediction = mu + sigma * sum_k w_k (m_k - mu) / sigma
@@ -27,7 +27,7 @@
 
 namespace todg_corr {
 
-constexpr int kGeo = 5, kVals = 16, kIn = kGeo + kVals, kH = 32, kOut = 16;
+constexpr int kGeo = 5, kVals = 16, kIn = kGeo + kVals, kH = 64, kOut = 16;
 // flat parameter layout (FP32): W1 (kH x kIn), b1, W2 (kH x kH), b2, W3 (kOut x kH), b3
 constexpr int kOffW1 = 0;
 constexpr int kOffB1 = kOffW1 + kH * kIn;
