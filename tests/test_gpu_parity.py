@@ -190,6 +190,28 @@ def test_spin2_guard_at_coincident_pixel(K, mode, variant):
     assert rel_err(g, r) <= TOL
 
 
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
+def test_footprint_centre_just_below_two_pi(K, mode):
+    """Boresights within half a step of the footprint's longitude grid (512
+    steps) below 2 pi, where the grid index wraps to 0: the footprint's
+    windows are then laid out from longitude origin 0 as well."""
+    nside = 256
+    rng = np.random.default_rng(71)
+    B, S = 48, 200
+    th = rng.uniform(0.3, 2.8, B)
+    ph = 2.0 * np.pi - rng.uniform(0.0, 0.95 * np.pi / 512, B)
+    ax = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1).astype(np.float32)
+    params = recenter_params(ax)
+    v = rng.standard_normal((S, 3)) * np.array([0.01, 0.01, 0.0]) + np.array([0.0, 0.0, 1.0])
+    vec = (v / np.linalg.norm(v, axis=-1, keepdims=True)).astype(np.float32)
+    mp = rng.uniform(-1.0, 1.0, (3, 12 * nside * nside)).astype(np.float32)
+    bv = rng.uniform(0.1, 1.0, S).astype(np.float32)
+    bv /= bv.sum()
+    g = _gpu_tod(K, mode, vec, params, nside, mp, bv, B, S, 1, 2)
+    r = _cpu_tod(mode, vec, params, nside, mp, bv, B, S, 1, 2)
+    assert rel_err(g, r) <= TOL
+
+
 def test_beam_tod_batch_dropin():
     """The reference's dispatcher contract end to end (numpy in, dict of
     float32 out) against the oracle."""

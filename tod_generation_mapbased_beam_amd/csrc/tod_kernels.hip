@@ -874,7 +874,13 @@ __device__ __forceinline__ void fp_layout(const TodArgs& a, int64_t b, const Bea
     float fphf = atan2f((float)fy, (float)fx);
     if (fphf < 0.f) fphf += 6.2831855f;
     int jl = (int)rintf(fphf * (float)(kLamN / kTwoPi));
-    jl = jl >= kLamN ? jl - kLamN : jl;
+    if (jl >= kLamN) {
+        // within half a grid step below 2 pi: lam = 0, and the windows'
+        // longitudes are measured from the same origin (a window start
+        // below 0 wraps like any other)
+        jl -= kLamN;
+        fphf -= 6.2831855f;
+    }
     const double lam = (double)jl * (kTwoPi / kLamN);
     const gptr<const double> trig = global_in(a.lam_trig);
     const double cl = trig[2 * jl], sl = trig[2 * jl + 1];
