@@ -139,7 +139,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the hot path "
-              "shards by detector x time chunk with no collectives, see DESIGN.md",
+              "would shard by detector x time chunk, each GPU holding a copy of the map, "
+              "with no collectives in the loop",
               file=sys.stderr)
         sys.exit(2)
 
