@@ -1888,7 +1888,7 @@ constexpr float kSpinF32St2 = 3e-4f;
 // 64 consecutive cells at a time, kP2Cells such chunks per round, so that
 // each lane has kP2Cells independent weight reads and gathers in flight.
 // Returns the per-thread sums (outside-grid sums included).
-constexpr int kP2Cells = 5;
+constexpr int kP2Cells = 4;
 
 template <bool QU, int CS>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
