@@ -71,6 +71,21 @@ int todg_gather_accum_fused(const float *vec_orig, const float *axes, const floa
                             const float *beam_vals, int64_t B, int64_t S, double *tod,
                             int c_q, int c_u, int variant, void *stream);
 
+/* variant (all three kernels below and todg_gather_accum): 0 = footprint
+ * kernel; 8 = the same for 2 waves/SIMD; 1/2/4 = direct kernel (bilinear and
+ * nearest); 16 + 3 i + j (bilinear and nearest) = footprint kernel with claims
+ * of {128, 256, 512}[i] beam pixels and {3, 4, 5}[j] 64-cell chunks per
+ * lane in the footprint sweep (variant 20 = variant 0's tile; built for I/Q/U
+ * with Q/U and for one field, other channel layouts run variant 0's tile).
+ * Any other value: status -1.  A sample whose rotation operands are not all
+ * finite is skipped (its tod entries are left as they are).  The same call
+ * repeated gives bit-identical results (weight-grid sums are exact, pixels
+ * outside a sample's footprint are summed in a fixed order), except for a
+ * sample whose laid-out footprint misses more than 32 of its neighbour
+ * pixels and, in bilinear mode, a sample whose footprint exceeds the grid
+ * (4608 cells or 128 rings: a beam over ~3.7 deg in radius at nside 2048):
+ * those agree to rounding. */
+
 /* Nearest-pixel beam-convolution TOD.  Replaces _gather_accum_nearest_jit
  * (tod_nearest.py:21-38, same argument order). */
 int todg_gather_accum_nearest(const float *vec_orig, const float *axes, const float *cos_a,
@@ -91,6 +106,35 @@ int todg_gather_accum_bicubic(const float *vec_orig, const float *axes, const fl
                               const float *sin_p, int64_t nside, const float *map_packed,
                               int C, const float *beam_vals, int64_t B, int64_t S,
                               double *tod, int c_q, int c_u, int variant, void *stream);
+
+/* The three kernels behind one entry (mode 0 = bilinear, 1 = nearest, 2 =
+ * bicubic), the C-ABI form of the reference's call surface sketched for this
+ * path (tod_core.py:129-164 dispatches to the kernels by interp_mode), with
+ * one more output: status (optional device int) gets bit 0 set when a
+ * sample's rotation operands were not all finite (that sample is skipped).
+ * The reference checks nothing (Numba kernels without bounds checks). */
+int todg_gather_accum(int mode, const float *vec_orig, const float *axes, const float *cos_a,
+                      const float *sin_a, const float *ax_pts, const float *cos_p,
+                      const float *sin_p, int64_t nside, const float *map_packed, int C,
+                      const float *beam_vals, int64_t B, int64_t S, double *tod, int c_q,
+                      int c_u, int variant, int *status, void *stream);
+
+/* Index dump of the production footprint kernel (variant 0's tile; mode 0 =
+ * bilinear, 1 = nearest; C = 3 with Q/U or C = 1): the same launch as
+ * todg_gather_accum (tod accumulated likewise) built to also record, for
+ * every (b, s), i = b S + s, what the kernel deposited: dump holds
+ * todg_fp_dump_bytes(B, S) bytes laid out as pix (4 B S int64: the bilinear
+ * neighbours in the order of numba_healpy.py:473-480, or the nearest pixel
+ * in pix[4 i] and -1 in the other three), w (4 B S float64: beam value x
+ * weight as deposited) and path (B S bytes: 0 = the certified fast path,
+ * 1 = the exact path).  B S <= 2^31.  Diagnostic entry (tests); the
+ * reference counterparts of the indices are _ring_interp_with_angles_jit
+ * (numba_healpy.py:358-499) and tod_nearest.py:108-141. */
+size_t todg_fp_dump_bytes(int64_t B, int64_t S);
+int todg_fp_dump(int mode, const float *vec_orig, const float *axes, const float *cos_a,
+                 const float *sin_a, const float *ax_pts, const float *cos_p, const float *sin_p,
+                 int64_t nside, const float *map_packed, int C, const float *beam_vals, int64_t B,
+                 int64_t S, double *tod, int c_q, int c_u, int64_t *dump, void *stream);
 
 /* Point values of one map field (map: npix float32, RING order) at N
  * directions (z, phi) (device float64 arrays): mode 0 = bilinear
@@ -126,18 +170,21 @@ int todg_gather_accum_batch(int mode, const todg_work_item *items, int n_items,
                             int C, int c_q, int c_u, int chunk, unsigned int *queue,
                             int *item_status, int variant, void *stream);
 
-/* Index-path dumps for samples [b0, b0+nb): the four bilinear neighbours and
- * weights of every (b, s) as evaluated inside todg_gather_accum_fused
- * (pix/wgt: (nb, S, 4) int64/float64), i.e. _rodrigues_apply_one_jit +
- * _ring_interp_with_angles_jit (tod_rotations.py:83-134,
- * numba_healpy.py:358-499). */
+/* Exact-path index dumps for samples [b0, b0+nb): the four bilinear
+ * neighbours and weights of every (b, s) as the kernels' exact path (the
+ * direct kernel, and the footprint kernel for the beam pixels its certified
+ * fast path does not take) evaluates them (pix/wgt: (nb, S, 4) int64/
+ * float64), i.e. _rodrigues_apply_one_jit + _ring_interp_with_angles_jit
+ * (tod_rotations.py:83-134, numba_healpy.py:358-499).  What the production
+ * footprint kernel itself deposits, fast path included: todg_fp_dump. */
 int todg_bilinear_indices(const float *vec_orig, const float *axes, const float *cos_a,
                           const float *sin_a, const float *ax_pts, const float *cos_p,
                           const float *sin_p, int64_t nside, int64_t B, int64_t S,
                           int64_t b0, int64_t nb, int64_t *pix, double *wgt, void *stream);
 
-/* The nearest pixel of every (b, s) as chosen inside todg_gather_accum_nearest
- * (pix: (nb, S) int64), i.e. tod_nearest.py:91-141. */
+/* The nearest pixel of every (b, s) as the nearest kernels' exact path
+ * chooses it (pix: (nb, S) int64), i.e. tod_nearest.py:91-141; the
+ * production footprint kernel's own choices: todg_fp_dump. */
 int todg_nearest_indices(const float *vec_orig, const float *axes, const float *cos_a,
                          const float *sin_a, const float *ax_pts, const float *cos_p,
                          const float *sin_p, int64_t nside, int64_t B, int64_t S,

@@ -1,0 +1,96 @@
+"""Argument checks of the Python entry points that run before any device work
+(CPU only: each call must raise ValueError without touching a GPU).
+
+A torch ``tod`` that is not a contiguous float64 CUDA tensor of shape (C, B)
+would hand a host pointer or a strided view to the kernels' atomic adds; the
+reference's contract is a (C, B) float64 array updated in place
+(tod_bilinear.py:220, tod_nearest.py:36).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from cases import random_rot_params
+
+
+@pytest.fixture(scope="module")
+def args():
+    rng = np.random.default_rng(0)
+    B, S, nside = 6, 5, 4
+    vec, params = random_rot_params(B, S, rng)
+    mp = rng.standard_normal((3, 12 * nside * nside)).astype(np.float32)
+    bv = np.full(S, 1.0 / S, np.float32)
+    return vec, params, nside, mp, bv, B, S
+
+
+BAD_TODS = {
+    "cpu tensor": lambda C, B: torch.zeros((C, B), dtype=torch.float64),
+    "float32": lambda C, B: np.zeros((C, B), np.float32),
+    "wrong shape": lambda C, B: np.zeros((C, B + 1)),
+    "transposed view": lambda C, B: torch.zeros((B, C), dtype=torch.float64).t(),
+    "list": lambda C, B: [[0.0] * B] * C,
+}
+
+
+@pytest.mark.parametrize("bad", sorted(BAD_TODS))
+@pytest.mark.parametrize("entry", ["_gather_accum_fused_jit", "_gather_accum_nearest_jit",
+                                   "_gather_accum_bicubic_jit"])
+def test_kernel_entries_reject_bad_tod(args, entry, bad):
+    from tod_generation_mapbased_beam_amd import kernels
+
+    vec, params, nside, mp, bv, B, S = args
+    tod = BAD_TODS[bad](3, B)
+    with pytest.raises(ValueError):
+        getattr(kernels, entry)(vec, *params, nside, mp, bv, B, S, tod, 1, 2)
+
+
+@pytest.mark.parametrize("bad", sorted(BAD_TODS))
+def test_gather_accum_rejects_bad_tod(args, bad):
+    from tod_generation_mapbased_beam_amd import kernels
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        kernels.gather_accum("bilinear", vec, *params, nside, mp, bv, B, S, BAD_TODS[bad](3, B), 1, 2)
+
+
+def test_gather_accum_rejects_bad_mode(args):
+    from tod_generation_mapbased_beam_amd import kernels
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        kernels.gather_accum("gaussian", vec, *params, nside, mp, bv, B, S, np.zeros((3, B)), 1, 2)
+
+
+@pytest.mark.parametrize("bad", ["cpu tensor", "transposed view"])
+def test_learned_entry_rejects_bad_tod(args, bad):
+    from tod_generation_mapbased_beam_amd import corrector
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        corrector.gather_accum_learned(vec, *params, nside, mp, bv, B, S, BAD_TODS[bad](3, B), 1, 2,
+                                       params=np.zeros(corrector.N_PARAMS, np.float32))
+
+
+def test_learned_entry_rejects_bad_stats(args):
+    from tod_generation_mapbased_beam_amd import corrector
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        corrector.gather_accum_learned(vec, *params, nside, mp, bv, B, S, np.zeros((3, B)), 1, 2,
+                                       params=np.zeros(corrector.N_PARAMS, np.float32),
+                                       stats=torch.zeros(2, dtype=torch.int64))
+
+
+def test_batch_plan_rejects_host_tod(args):
+    from tod_generation_mapbased_beam_amd.batch import BatchPlan, Detector
+
+    vec, params, nside, mp, bv, B, S = args
+
+    class FakeMap:  # the plan checks tods before it uploads anything
+        C, c_q, c_u, nside = 3, 1, 2, 4
+
+    for bad in ("cpu tensor", "transposed view"):
+        det = Detector(vec, bv, params, tod=BAD_TODS[bad](3, B))
+        with pytest.raises(ValueError):
+            BatchPlan([det], FakeMap())

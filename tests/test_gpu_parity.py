@@ -110,14 +110,124 @@ def test_direct_tile_shapes_identical(K, tile):
     assert np.array_equal(ref, got)
 
 
-def test_footprint_deterministic(K):
-    """Same inputs, same footprint-kernel result, up to LDS summation order."""
+@pytest.mark.parametrize("variant", [0, 8, 16, 24])
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
+def test_footprint_deterministic(K, mode, variant):
+    """Same inputs, same footprint-kernel result bit for bit: weight-grid
+    deposits are rounded onto a grid on which their sums are exact, so the
+    order of the LDS atomics does not matter (the reference's result is
+    thread-count independent, tests/test_tod_bilinear.py:830-870)."""
     c = small_config()
     B = 3001
     params = tuple(p[:B] for p in c["params"])
-    a = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2)
-    b = _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2)
-    assert rel_err(a, b) <= 1e-12
+    a = _gpu_tod(K, mode, c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2, variant)
+    b = _gpu_tod(K, mode, c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2, variant)
+    assert np.array_equal(a, b)
+    r = _cpu_tod(mode, c["vec"], params, c["nside"], c["map"], c["bv"], B, c["S"], 1, 2)
+    assert rel_err(a, r) <= TOL
+
+
+@pytest.mark.parametrize("variant", list(range(16, 25)))
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
+@pytest.mark.parametrize("qu", [(1, 2), (-1, -1)])
+def test_tile_variants_tod(K, mode, variant, qu):
+    """Every tile variant (claims of 128/256/512 beam pixels x 3/4/5 sweep
+    chunks) against the oracle, for both channel layouts they are built for
+    and for one they fall back from (C = 3 without Q/U)."""
+    c = small_config()
+    B = 1500
+    params = tuple(p[:B] for p in c["params"])
+    mp = c["map"] if qu != (-1, -1) else c["map"][:1]
+    for m in (mp, c["map"]) if qu == (-1, -1) else (mp,):
+        g = _gpu_tod(K, mode, c["vec"], params, c["nside"], m, c["bv"], B, c["S"], *qu, variant)
+        r = _cpu_tod(mode, c["vec"], params, c["nside"], m, c["bv"], B, c["S"], *qu)
+        assert rel_err(g, r) <= TOL
+
+
+def test_unknown_variant_rejected(K):
+    c = small_config()
+    params = tuple(p[:10] for p in c["params"])
+    with pytest.raises(ValueError):
+        _gpu_tod(K, "bilinear", c["vec"], params, c["nside"], c["map"], c["bv"], 10, c["S"], 1, 2, 3)
+
+
+@pytest.mark.parametrize("mode", ["bilinear", "nearest", "bicubic"])
+@pytest.mark.parametrize("variant", [0, 1])
+def test_non_finite_pointing_skipped_and_flagged(K, mode, variant):
+    """A sample with a non-finite rotation operand is skipped (its tod
+    entries keep their values), the others are computed as without it, and
+    todg_gather_accum's status word says so."""
+    import torch
+
+    if mode == "bicubic" and variant == 1:
+        pytest.skip("the bicubic judge runs on the footprint kernel only")
+    c = small_config()
+    B = 600
+    params = [p[:B].copy() for p in c["params"]]
+    bad = [0, 1, 77, 300, 599]
+    params[0][bad[0], 1] = np.nan
+    params[1][bad[1]] = np.inf
+    params[3][bad[2], 2] = np.nan
+    params[4][bad[3]] = -np.inf
+    params[5][bad[4]] = np.nan
+    good = np.setdiff1d(np.arange(B), bad)
+    tod = np.full((3, B), 5.0)
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    K.gather_accum(mode, c["vec"], *params, c["nside"], c["map"], c["bv"], B, c["S"], tod, 1, 2,
+                   variant=variant, status=status)
+    assert int(status.item()) & 1
+    assert np.all(tod[:, bad] == 5.0)
+    ref = np.zeros((3, good.size))
+    sub = [p[good] for p in params]
+    fn = {"bilinear": O.gather_accum_fused, "nearest": O.gather_accum_nearest,
+          "bicubic": O.gather_accum_bicubic}[mode]
+    fn(c["vec"], *sub, c["nside"], c["map"], c["bv"], good.size, c["S"], ref, 1, 2)
+    assert rel_err(tod[:, good] - 5.0, ref) <= TOL
+    status.zero_()
+    ok = [p[good] for p in params]
+    t2 = np.zeros((3, good.size))
+    K.gather_accum(mode, c["vec"], *ok, c["nside"], c["map"], c["bv"], good.size, c["S"], t2, 1, 2,
+                   variant=variant, status=status)
+    assert int(status.item()) == 0
+
+
+def test_fallback_without_mp_stacked():
+    """beam_tod_batch without mp_stacked (the reference's healpy split
+    pipeline, tod_core.py:169-180: batch Rodrigues with float32 rotated
+    vectors, vec2ang, get_interp_weights, no spin-2) against the oracle's
+    restatement of that pipeline, to the reference's own tolerance for the
+    split-vs-fused rounding gap (1e-5 absolute,
+    tests/test_tod_bilinear.py:874-914).  vec2ang is healpy's (absent here):
+    theta = arccos(z / |v|), phi = atan2(y, x) wrapped to [0, 2 pi)."""
+    from tod_generation_mapbased_beam_amd import beam_tod_batch
+    from tod_generation_mapbased_beam_amd import tod_rotations as R
+
+    rng = np.random.default_rng(45)
+    for nside in (4, 16, 64):
+        B, S, C = 8, 20, 3
+        vec, _ = random_rot_params(B, S, rng)
+        th = rng.uniform(0.2, np.pi - 0.2, B).astype(np.float32)
+        ph = rng.uniform(0.0, 2 * np.pi, B).astype(np.float32)
+        ps = rng.uniform(0.0, 2 * np.pi, B).astype(np.float32)
+        rot, beta = R.precompute_rotation_vector_batch(np.zeros((1, 1)), np.full((1, 1), np.pi / 2),
+                                                       ph, th, center_idx=(0, 0))
+        psis = -beta + ps
+        mp = rng.uniform(0.5, 1.5, (C, 12 * nside * nside)).astype(np.float32)
+        bv = rng.uniform(0.1, 1.0, S).astype(np.float32)
+        bv /= bv.sum()
+        data = dict(vec_orig=vec, beam_vals=bv, comp_indices=[0, 1, 2])
+        out = beam_tod_batch(nside, mp, data, rot, ph, th, psis)
+        got = np.stack([out[k] for k in (0, 1, 2)]).astype(np.float64)
+        params = R._rotation_params(rot, ph, th, psis)
+        vr = O.rodrigues_batch(vec, *params).reshape(-1, 3).astype(np.float64)
+        r = np.linalg.norm(vr, axis=1)
+        theta = np.arccos(vr[:, 2] / r)
+        phi = np.arctan2(vr[:, 1], vr[:, 0])
+        phi[phi < 0] += 2 * np.pi
+        pix, wgt = O.get_interp_weights(nside, theta, phi)
+        ref = np.zeros((C, B))
+        O.gather_accum(pix, wgt, bv, mp, B, S, ref)
+        np.testing.assert_allclose(got, ref, atol=1e-5)
 
 
 def test_accumulates_in_place(K):

@@ -75,6 +75,16 @@ SIGNATURES = {
     "todg_gather_accum_batch": (
         _i32, [_i32, _p, _i32, _i64, _i64, _p, _i32, _i32, _i32, _i32, _p, _p, _i32, _p],
     ),
+    "todg_gather_accum": (
+        _i32,
+        [_i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _i32, _p,
+         _p],
+    ),
+    "todg_fp_dump_bytes": (ctypes.c_size_t, [_i64, _i64]),
+    "todg_fp_dump": (
+        _i32,
+        [_i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _p, _p],
+    ),
 }
 
 

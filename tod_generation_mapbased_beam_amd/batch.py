@@ -48,6 +48,13 @@ class BatchPlan:
         self.dmap = dmap
         self.chunk = int(chunk)
         self.variant = int(variant)
+        from .kernels import check_tod
+
+        for i, d in enumerate(detectors):  # before anything is uploaded
+            if d.tod is not None:
+                if not isinstance(d.tod, torch.Tensor):
+                    raise ValueError(f"detector {i}: tod must be a CUDA tensor (or None)")
+                check_tod(d.tod, dmap.C, int(d.params[1].shape[0]), f"detector {i} tod")
         dev = device()
         f32 = torch.float32
         self._keep = []  # device tensors referenced by the item array
@@ -63,8 +70,6 @@ class BatchPlan:
             if S > (1 << 28):
                 raise ValueError(f"detector {i}: at most 2^28 beam pixels")
             tod = d.tod if d.tod is not None else torch.zeros((dmap.C, B), dtype=torch.float64, device=dev)
-            if tod.dtype != torch.float64 or tuple(tod.shape) != (dmap.C, B) or not tod.is_contiguous():
-                raise ValueError(f"detector {i}: tod must be contiguous float64 ({dmap.C}, {B})")
             self._keep += [v, bv, *ps, tod]
             self.tods.append(tod)
             rows[i] = (v.data_ptr(), bv.data_ptr(), *(p.data_ptr() for p in ps), tod.data_ptr(),

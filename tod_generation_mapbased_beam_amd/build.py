@@ -16,6 +16,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO_DIR = os.path.dirname(PKG_DIR)
@@ -23,8 +24,9 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 LIB_NAME = "libtodgpu.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
-SOURCES = ["tod_kernels.hip", "corrector.hip", "beam_tools.hip"]
-HEADERS = ["healpix_ring.hpp", "internal.hpp", "corrector.hpp"]
+SOURCES = ["tod_fp_bilinear.hip", "tod_fp_nearest.hip", "tod_fp_bicubic.hip", "tod_fp_learned.hip",
+           "tod_kernels.hip", "corrector.hip", "beam_tools.hip"]
+HEADERS = ["healpix_ring.hpp", "internal.hpp", "corrector.hpp", "tod_device.hpp"]
 
 HIPCC_FLAGS = [
     "-O3",
@@ -32,13 +34,13 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-ffp-contract=off",
     "-fPIC",
-    "-shared",
     "-Xclang",
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
 ]
+OBJ_DIR = os.path.join(PKG_DIR, "_build")
 
 
 def hipcc() -> str:
@@ -48,16 +50,28 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
-    """Compile libtodgpu.so; returns its path.  Always recompiles."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(REPO_DIR, "include"), "-o", tmp, *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compile libtodgpu.so (one object per source, compiled in parallel,
+    then linked); returns its path.  Always recompiles."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    inc = ["-I", os.path.join(REPO_DIR, "include")]
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [[hipcc(), *HIPCC_FLAGS, *inc, "-c", "-o", o, os.path.join(CSRC, s)]
+            for s, o in zip(SOURCES, objs)]
+    jobs = jobs or min(len(cmds), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for f in [pool.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    tmp = LIB_PATH + ".tmp"
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

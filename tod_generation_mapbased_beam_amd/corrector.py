@@ -216,10 +216,16 @@ def gather_accum_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nsi
     ``stats``: optional int64 device tensor (2,) that receives (corrected,
     plain bilinear) beam-pixel-sample counts."""
     from .device import device_map_for
+    from .kernels import _map_channels, check_tod
 
     B, S, c_q, c_u = int(B), int(S), int(c_q), int(c_u)
     if not 0.0 < float(power) <= 1.0:
         raise ValueError("power must be in (0, 1]")
+    check_tod(tod, _map_channels(mp_stacked), B)
+    if stats is not None and not (isinstance(stats, torch.Tensor) and stats.device.type == "cuda"
+                                  and stats.dtype == torch.int64 and stats.numel() >= 2
+                                  and stats.is_contiguous()):
+        raise ValueError("stats must be a contiguous CUDA int64 tensor with at least 2 elements")
     dmap = device_map_for(mp_stacked, c_q, c_u)
     if dmap.nside != int(nside):
         raise ValueError(f"map nside {dmap.nside} != nside {nside}")
@@ -230,8 +236,6 @@ def gather_accum_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nsi
         raise ValueError(f"expected {N_PARAMS} corrector parameters")
     host = isinstance(tod, np.ndarray)
     t = torch.from_numpy(np.ascontiguousarray(tod)).to(ops[0].device) if host else tod
-    if t.dtype != torch.float64 or tuple(t.shape) != (dmap.C, B):
-        raise ValueError(f"tod must be float64 of shape ({dmap.C}, {B})")
     v, ax, ca, sa, bp, cp, sp, bv = ops
     n_corr = S
     if power < 1.0:

@@ -39,6 +39,8 @@ def to_dev(x, dtype: torch.dtype) -> torch.Tensor:
             t = t.to(dtype)
         return t.contiguous()
     arr = np.ascontiguousarray(np.asarray(x), dtype=torch.empty((), dtype=dtype).numpy().dtype)
+    if not arr.flags.writeable:  # torch.from_numpy wants a writeable buffer
+        arr = arr.copy()
     return torch.from_numpy(arr).to(device())
 
 
@@ -84,10 +86,13 @@ class DeviceMap:
         return (self.C, self.c_q, self.c_u) == (C, c_q, c_u)
 
 
-# Packed copies of host (numpy) maps, keyed by the array object (ndarrays
-# are unhashable, so by id() with a weak reference that drops the entry when
-# the array dies): the reference's production maps are read-only for the
-# life of the run.
+# Packed copies of READ-ONLY host (numpy) maps, keyed by the array object
+# (ndarrays are unhashable, so by id() with a weak reference that drops the
+# entry when the array dies).  A writeable array is packed again on every
+# call, so the kernels always see its current values, as the reference's
+# kernels read the live array; callers that reuse one map for many calls mark
+# it read-only (arr.setflags(write=False), as the driver's beam entries are)
+# or pass a DeviceMap.
 _host_cache: dict = {}
 
 
@@ -106,7 +111,7 @@ def device_map_for(mp_stacked, c_q: int, c_u: int) -> DeviceMap:
         if mp_stacked.c_q != c_q or mp_stacked.c_u != c_u:
             raise ValueError("DeviceMap was packed for a different (c_q, c_u)")
         return mp_stacked
-    if isinstance(mp_stacked, np.ndarray):
+    if isinstance(mp_stacked, np.ndarray) and not mp_stacked.flags.writeable:
         per = _host_entry(mp_stacked)
         key = (c_q, c_u, mp_stacked.__array_interface__["data"][0], mp_stacked.shape)
         if key not in per:

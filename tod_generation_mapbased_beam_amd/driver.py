@@ -43,11 +43,13 @@ def beam_entry(vec_orig, beam_vals, comp_indices, mp, ra=None, dec=None):
     for beams prepared in memory: ``mp_stacked`` holds the entry's components
     of the (3, npix) map as float32."""
     comp = list(comp_indices)
+    stacked = np.ascontiguousarray(np.stack([np.asarray(mp[c]) for c in comp]), np.float32)
+    stacked.setflags(write=False)  # read-only: its packed device copy is made once and reused
     return {
         "vec_orig": np.ascontiguousarray(vec_orig, np.float32),
         "beam_vals": np.ascontiguousarray(beam_vals, np.float32),
         "comp_indices": comp,
-        "mp_stacked": np.ascontiguousarray(np.stack([np.asarray(mp[c]) for c in comp]), np.float32),
+        "mp_stacked": stacked,
         "ra": np.zeros((1, 1)) if ra is None else ra,
         "dec": np.full((1, 1), np.pi / 2) if dec is None else dec,
         "n_sel": int(np.asarray(beam_vals).shape[0]),

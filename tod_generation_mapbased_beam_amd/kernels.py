@@ -22,12 +22,45 @@ from .device import device_map_for, stream_handle, to_dev
 DEFAULT_VARIANT = 0
 
 
+def _map_channels(mp_stacked) -> int:
+    """C of a map argument (a DeviceMap, or a (C, npix) / (npix,) array)."""
+    if hasattr(mp_stacked, "packed") and hasattr(mp_stacked, "C"):
+        return int(mp_stacked.C)
+    shape = tuple(mp_stacked.shape)
+    return 1 if len(shape) == 1 else int(shape[0])
+
+
+def check_tod(tod, C: int, B: int, name: str = "tod") -> None:
+    """The reference's accumulator contract (tod_bilinear.py:220, (C, B)
+    float64, updated in place): a numpy array, or a contiguous CUDA tensor on
+    the current device.  Checked before anything touches the device."""
+    if isinstance(tod, np.ndarray):
+        if tod.dtype != np.float64 or tod.shape != (C, B):
+            raise ValueError(f"{name} must be float64 of shape ({C}, {B})")
+        return
+    if not isinstance(tod, torch.Tensor):
+        raise ValueError(f"{name} must be a numpy array or a torch tensor")
+    if tod.dtype != torch.float64 or tuple(tod.shape) != (C, B):
+        raise ValueError(f"{name} must be a float64 tensor of shape ({C}, {B})")
+    if not tod.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if tod.device.type != "cuda":
+        raise ValueError(f"{name} must live on the GPU (got a {tod.device.type} tensor); "
+                         "pass a numpy array to have it copied in and out")
+    if tod.device.index is not None and tod.device.index != torch.cuda.current_device():
+        raise ValueError(f"{name} is on cuda:{tod.device.index}, not the current device "
+                         f"cuda:{torch.cuda.current_device()}")
+
+
 def _run(entry, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_stacked,
-         beam_vals, B, S, tod, c_q, c_u, variant):
+         beam_vals, B, S, tod, c_q, c_u, variant, mode=None, status=None):
     B = int(B)
     S = int(S)
     c_q = int(c_q)
     c_u = int(c_u)
+    check_tod(tod, _map_channels(mp_stacked), B)
+    if status is not None:
+        check_status(status)
     dmap = device_map_for(mp_stacked, c_q, c_u)
     if dmap.nside != int(nside):
         raise ValueError(f"map nside {dmap.nside} != nside {nside}")
@@ -40,24 +73,43 @@ def _run(entry, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_st
         raise ValueError("rotation operands shorter than B")
     host_tod = None
     if isinstance(tod, np.ndarray):
-        if tod.dtype != np.float64 or tod.shape != (dmap.C, B):
-            raise ValueError(f"tod must be float64 of shape ({dmap.C}, {B})")
         host_tod = tod
         t = torch.from_numpy(np.ascontiguousarray(tod)).to(v.device)
     else:
         t = tod
-        if t.dtype != torch.float64 or not t.is_contiguous() or tuple(t.shape) != (dmap.C, B):
-            raise ValueError(f"tod must be a contiguous float64 tensor of shape ({dmap.C}, {B})")
-    _lib.check(
-        entry(
-            _lib.ptr(v), _lib.ptr(ax), _lib.ptr(ca), _lib.ptr(sa), _lib.ptr(bp), _lib.ptr(cp),
+    args = (_lib.ptr(v), _lib.ptr(ax), _lib.ptr(ca), _lib.ptr(sa), _lib.ptr(bp), _lib.ptr(cp),
             _lib.ptr(sp), int(nside), _lib.ptr(dmap.packed), dmap.C, _lib.ptr(bv), B, S,
-            _lib.ptr(t), c_q, c_u, int(variant), stream_handle(),
-        )
-    )
+            _lib.ptr(t), c_q, c_u, int(variant))
+    if mode is None:
+        _lib.check(entry(*args, stream_handle()))
+    else:
+        _lib.check(entry(int(mode), *args, _lib.ptr(status), stream_handle()))
     if host_tod is not None:
         host_tod[...] = t.cpu().numpy()
     return tod
+
+
+def check_status(status) -> None:
+    if not (isinstance(status, torch.Tensor) and status.dtype == torch.int32 and status.numel() >= 1
+            and status.device.type == "cuda"):
+        raise ValueError("status must be a CUDA int32 tensor with at least one element")
+
+
+MODES = {"bilinear": 0, "nearest": 1, "bicubic": 2}
+
+
+def gather_accum(interp_mode, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_stacked,
+                 beam_vals, B, S, tod, c_q=-1, c_u=-1, variant=DEFAULT_VARIANT, status=None):
+    """Any of the three kernels (``todg_gather_accum``), accumulating into
+    ``tod``; a sample with a non-finite rotation operand is skipped (its tod
+    entries keep their values) and sets bit 0 of ``status`` (a CUDA int32
+    tensor, or None)."""
+    if interp_mode not in MODES:
+        raise ValueError(f"interp_mode {interp_mode!r} not available; use 'bilinear', 'nearest' "
+                         "or 'bicubic'")
+    return _run(_lib.lib().todg_gather_accum, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p,
+                nside, mp_stacked, beam_vals, B, S, tod, c_q, c_u, variant,
+                mode=MODES[interp_mode], status=status)
 
 
 def _gather_accum_fused_jit(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside,
@@ -125,6 +177,39 @@ def bilinear_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, 
         )
     )
     return pix.cpu().numpy(), wgt.cpu().numpy()
+
+
+def fp_dump(interp_mode, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_stacked,
+            beam_vals, B, S, c_q=-1, c_u=-1):
+    """Run the production footprint kernel's index-dump build over all B
+    samples: returns (pix (B, S, 4) int64, w (B, S, 4) float64, path (B, S)
+    uint8, tod (C, B) float64) -- the pixels each beam pixel was deposited
+    into (bilinear neighbours in the reference's order; nearest: column 0),
+    the deposited beam value x weight, the path that took it (0 certified
+    fast path, 1 exact path) and the TOD of the same launch."""
+    if interp_mode not in ("bilinear", "nearest"):
+        raise ValueError("index dumps exist for 'bilinear' and 'nearest'")
+    B = int(B)
+    S = int(S)
+    c_q = int(c_q)
+    c_u = int(c_u)
+    dmap = device_map_for(mp_stacked, c_q, c_u)
+    f32 = torch.float32
+    ops = [to_dev(a, f32) for a in (vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, beam_vals)]
+    v, ax, ca, sa, bp, cp, sp, bv = ops
+    nbytes = int(_lib.lib().todg_fp_dump_bytes(B, S))
+    buf = torch.empty(max(nbytes, 8) // 8 + 1, dtype=torch.int64, device=v.device)
+    tod = torch.zeros((dmap.C, B), dtype=torch.float64, device=v.device)
+    _lib.check(_lib.lib().todg_fp_dump(
+        MODES[interp_mode], _lib.ptr(v), _lib.ptr(ax), _lib.ptr(ca), _lib.ptr(sa), _lib.ptr(bp),
+        _lib.ptr(cp), _lib.ptr(sp), int(nside), _lib.ptr(dmap.packed), dmap.C, _lib.ptr(bv), B, S,
+        _lib.ptr(tod), c_q, c_u, _lib.ptr(buf), stream_handle()))
+    n = B * S
+    raw = buf.cpu().numpy()
+    pix = raw[: 4 * n].reshape(B, S, 4)
+    w = raw[4 * n: 8 * n].view(np.float64).reshape(B, S, 4)
+    path = raw.view(np.uint8)[64 * n: 65 * n].reshape(B, S)
+    return pix, w, path, tod.cpu().numpy()
 
 
 def nearest_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, B, S, b0, nb):
