@@ -70,8 +70,13 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     with ThreadPoolExecutor(max_workers=jobs) as pool:
         for f in [pool.submit(_run, c, verbose) for c in cmds]:
             f.result()
+    # export the C ABI (todg_*) only; the C++ internals stay local
+    vscript = os.path.join(OBJ_DIR, "exports.map")
+    with open(vscript, "w") as f:
+        f.write("{ global: todg_*; local: *; };\n")
     tmp = LIB_PATH + ".tmp"
-    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", f"-Wl,--version-script={vscript}", "-o",
+          tmp, *objs], verbose)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
