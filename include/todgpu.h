@@ -207,6 +207,18 @@ size_t todg_beam_bell_workspace(int64_t N, int lmax);
 int todg_beam_bell(const double *x, const double *w, int64_t N, int lmax, double *ws,
                    double *bell, void *stream);
 
+/* todg_beam_bell without a workspace argument, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight loops such as the clustering
+ * calibration sweep).  The per-block partial sums live in a library-owned
+ * device array of 2^20 doubles (8 MiB on each device it runs on, held until
+ * the process ends); calls with ceil(N / 256) * (lmax + 1) beyond that return
+ * TODG_ERR_ARG.  Every call on one device shares that array: calls that
+ * overlap on one device (different streams, or host threads) race.  Results
+ * are bit-identical to todg_beam_bell's. */
+int todg_beam_bell_noalloc(const double *x, const double *w, int64_t N, int lmax, double *bell,
+                           void *stream);
+
 /* ------------------------------------------------------------------------
  * Learned interpolation corrector (no counterpart in the reference: the
  * build's corrector of bilinear interpolation toward the bicubic judge).

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 
 #include "internal.hpp"
@@ -69,7 +70,51 @@ __global__ __launch_bounds__(256) void k_bell_sum(const double* __restrict__ par
     bell[l] = s;
 }
 
+__global__ __launch_bounds__(256) void k_fill_zero(double* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// Scratch of todg_beam_bell_noalloc: the per-block partial sums, at most
+// kScratchDoubles of them (8 MiB per device, held until the process ends).
+constexpr int64_t kScratchDoubles = int64_t(1) << 20;
+
 }  // namespace todg_beam
+
+// The no-allocation entry's scratch.  In host code this name stands for the
+// array but is not its device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device (scratch_address).
+static __device__ __align__(256) double scratch[todg_beam::kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+double* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+
+// The current device's address of `scratch`, looked up once per device (with
+// that device current) and kept.  No allocation: the array lives in the
+// library's code object on each device.
+int scratch_address(double** out) {
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return todg_internal::set_error(TODG_ERR_HIP, hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices)
+        return todg_internal::set_error(TODG_ERR_ARG, "device index beyond 63");
+    std::call_once(g_scratch_once[dev], [dev] {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    });
+    if (g_scratch_err[dev] != hipSuccess)
+        return todg_internal::set_error(TODG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                                          hipGetErrorString(g_scratch_err[dev]));
+    *out = g_scratch_ptr[dev];
+    return TODG_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -95,6 +140,40 @@ int todg_beam_bell(const double* x, const double* w, int64_t N, int lmax, double
     hipLaunchKernelGGL(todg_beam::k_bell_sum, dim3((unsigned)((lmax + 256) / 256)), dim3(256), 0, st,
                        ws, (int)nb, lmax, bell);
     hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : todg_internal::set_error(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_beam_bell_noalloc(const double* x, const double* w, int64_t N, int lmax, double* bell,
+                           void* stream) {
+    if (N < 0 || lmax < 0) return todg_internal::set_error(TODG_ERR_ARG, "N and lmax must be >= 0");
+    if (!bell || (N > 0 && (!x || !w)))
+        return todg_internal::set_error(TODG_ERR_ARG, "null pointer argument");
+    const int64_t nb = (N + todg_beam::kThreads - 1) / todg_beam::kThreads;
+    const int64_t need = (nb > 0 ? nb : 1) * (int64_t)(lmax + 1);
+    if (need > todg_beam::kScratchDoubles)
+        return todg_internal::set_error(
+            TODG_ERR_ARG, "todg_beam_bell_noalloc: N = " + std::to_string(N) + " and lmax = " +
+                              std::to_string(lmax) + " need " + std::to_string(need) +
+                              " doubles of scratch, more than its " +
+                              std::to_string(todg_beam::kScratchDoubles) +
+                              " (ceil(N / 256) * (lmax + 1) must stay within it; todg_beam_bell "
+                              "takes a caller workspace of any size)");
+    double* ws = nullptr;
+    const int rc = scratch_address(&ws);
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (N == 0) {  // a fill kernel rather than a memset node, so graph capture replays it
+        hipLaunchKernelGGL(todg_beam::k_fill_zero, dim3((unsigned)((lmax + 256) / 256)), dim3(256), 0, st,
+                           bell, lmax + 1);
+    } else {
+        // k_bell_partial writes every scratch entry that k_bell_sum reads:
+        // nothing needs clearing first
+        hipLaunchKernelGGL(todg_beam::k_bell_partial, dim3((unsigned)nb), dim3(todg_beam::kThreads), 0, st,
+                           x, w, N, lmax, ws);
+        hipLaunchKernelGGL(todg_beam::k_bell_sum, dim3((unsigned)((lmax + 256) / 256)), dim3(256), 0, st,
+                           ws, (int)nb, lmax, bell);
+    }
+    const hipError_t e = hipGetLastError();
     return e == hipSuccess ? TODG_OK : todg_internal::set_error(TODG_ERR_HIP, hipGetErrorString(e));
 }
 
