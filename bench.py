@@ -9,11 +9,19 @@ correction.  One step = one full pass of the hot path over the day: zero the
 B x S beam-pixel samples.  Inputs are resident in HBM before timing starts.
 
 Prints ONE JSON line.  ``value`` is beam-pixel.samples per second (B*S per
-step / step time).  ``roofline`` prices the kernel against HBM bandwidth
-using the algorithmic gather bytes (4 neighbours x 3 fields x 4 B = 48 B per
-beam-pixel.sample).  ``cpu_baseline`` times the CPU oracle (the C
-restatement of the reference's Numba kernel, OpenMP over samples) on a
-bounded contiguous subset of the same workload on this host's cores.
+step / step time).  ``roofline`` prices the kernel against the resource that
+bounds it, which the hardware counters show is VALU issue, not HBM: the
+kernel's VALU wave-instructions per launch (rocprofv3 SQ_INSTS_VALU, from the
+committed counter summary of this workload) over its live kernel time,
+against MI355X's peak issue rate (1024 SIMDs x 2.4 GHz / 4 cycles per wave64
+instruction); the measured HBM traffic (FETCH_SIZE + WRITE_SIZE, copy-
+calibrated) and its fraction of the 8 TB/s peak are reported beside it, as is
+the nominal gather rate of 48 algorithmic bytes per beam-pixel.sample (4
+neighbours x 3 fields x 4 B), which is not a bound: each footprint pixel is
+gathered once per sample and mostly from L2.  ``cpu_baseline`` times the CPU
+oracle (the C restatement of the reference's Numba kernel, OpenMP over
+samples) on a bounded contiguous subset of the same workload on this host's
+cores.
 """
 
 from __future__ import annotations
@@ -32,34 +40,65 @@ sys.path.insert(0, REPO)
 METRIC = "beam-pix·samples/s at nside 2048; learned vs bilinear at bicubic-judged RMS err"
 UNIT = "beam-pix·samples/s"
 HBM_PEAK_GBS = 8000.0
+# MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one wave64 VALU
+# instruction per SIMD every 4 cycles (16 lanes; binary32 and binary64 alike)
+VALU_PEAK_GINST = 1024 * 2.4 / 4.0
 # neighbours x 3 fields x float32 (learned: the corrector's 4 x 4 neighbourhood)
 BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 192}
+DTYPE = "mixed: f64 rotation, index path and accumulation; f32 map, weights and near-boresight spin-2"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "k_tod_fp_bilinear_nside2048_S6364_B1728000_counters.md")
+PROFILES = os.path.join(REPO, "profiles")
 
 
-def measured_traffic(mode, nside, S, B, variant):
-    """HBM bytes per launch of the dominant kernel from the committed counter
-    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when it covers this
-    exact workload; None otherwise."""
-    try:
-        lines = open(TRAFFIC_PROFILE).read().splitlines()
-    except OSError:
-        return None
-    kv = {}
-    for ln in lines:
-        if " = " in ln:
-            k, v = ln.split(" = ", 1)
-            kv[k.strip()] = v.strip()
-    want = f"{mode} nside {nside} S {S} B {B} variant {variant}"
-    if kv.get("workload") != want or "traffic_bytes_per_launch" not in kv:
-        return None
-    return float(kv["traffic_bytes_per_launch"])
+def counter_summary(workload):
+    """key = value lines of the committed counter summary whose `workload`
+    line matches, or {} (profiles/*_counters.md)."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(PROFILES, "*_counters.md"))):
+        kv = {}
+        for ln in open(path).read().splitlines():
+            if " = " in ln and not ln.startswith(" "):
+                k, v = ln.split(" = ", 1)
+                kv[k.strip()] = v.strip()
+        if kv.get("workload") == workload:
+            kv["_file"] = os.path.relpath(path, REPO)
+            return kv
+    return {}
+
+
+def roofline(workload, kern_ms, work, bytes_per_bps, variant=None):
+    """The roofline object of one bench line: VALU-issue bound from the
+    committed counters of this workload (None fields when none match)."""
+    kv = counter_summary(workload)
+    sec = kern_ms / 1e3
+    out = {"bound": "valu", "achieved": None, "peak": VALU_PEAK_GINST,
+           "unit": "G wave64 VALU instructions/s", "frac": None, "traffic": None,
+           "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+           "hbm_GBs": None, "hbm_frac": None,
+           "nominal_gather_GBs": work * bytes_per_bps / sec / 1e9,
+           "nominal_gather_note": f"{bytes_per_bps} algorithmic map bytes per beam-pixel.sample over the kernel "
+                                  "time; not a bound (footprint pixels are gathered once per sample, mostly "
+                                  "from L2)",
+           "counters": kv.get("_file"), "counters_variant": kv.get("variant")}
+    if variant is not None and kv.get("variant") not in (None, str(variant)):
+        out["counters_note"] = (f"instruction and traffic counts measured on variant {kv.get('variant')}, "
+                                f"timed variant {variant}")
+    if "valu_insts_per_launch" in kv:
+        ach = float(kv["valu_insts_per_launch"]) / sec / 1e9
+        out["achieved"] = ach
+        out["frac"] = ach / VALU_PEAK_GINST
+    if "traffic_bytes_per_launch" in kv:
+        t = float(kv["traffic_bytes_per_launch"])
+        out["traffic"] = t
+        out["hbm_GBs"] = t / sec / 1e9
+        out["hbm_frac"] = out["hbm_GBs"] / HBM_PEAK_GBS
+    return out
 
 
 def build_workload(nside, nside_beam, fwhm, n_samples, fs, seed, device):
@@ -117,8 +156,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--variant", type=int, default=0,
-                    help="0 = footprint kernel, 1/2/4 = direct kernel with that sample tile")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="-1 (default): the autotuner's choice for this workload (timed on a prefix "
+                         "before the warmup); 0 = footprint kernel, 8 = its 2-waves/SIMD build, "
+                         "16..24 = its tile variants, 1/2/4 = direct kernel with that sample tile")
     ap.add_argument("--nside", type=int, default=2048)
     ap.add_argument("--nside-beam", type=int, default=4096)
     ap.add_argument("--fwhm", type=float, default=30.0)
@@ -162,6 +203,18 @@ def main():
     dbv = to_dev(bv, f32)
     dparams = [to_dev(p, f32) for p in params]
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
+    variant, variant_note = args.variant, "given"
+    if variant < 0:
+        if args.mode == "learned":
+            variant, variant_note = 0, "the corrector has one build"
+        else:
+            from tod_generation_mapbased_beam_amd import autotune
+
+            variant, tms = autotune.tune_single(dvec, dbv, dparams, dmap, mode=args.mode,
+                                                max_samples=131072, reps=3)
+            variant_note = "autotuned on the first 131072 samples: " + ", ".join(
+                f"{k}: {v:.2f} ms" for k, v in sorted(tms.items()))
+            log(f"autotune: variant {variant} ({variant_note})")
     if args.mode == "learned":
         from tod_generation_mapbased_beam_amd import corrector
 
@@ -176,7 +229,7 @@ def main():
         tod.zero_()
         if ev0 is not None:
             ev0.record()
-        fn(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, variant=args.variant)
+        fn(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, variant=variant)
         if ev1 is not None:
             ev1.record()
 
@@ -202,9 +255,10 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     work = float(B) * S
     value = work / (ms_per_step / 1e3)
-    achieved = work * BYTES_PER_BPS[args.mode] / (kern_ms / 1e3) / 1e9
+    roof = roofline(f"single {args.mode} nside {args.nside} S {S} B {B}", kern_ms, work,
+                    BYTES_PER_BPS[args.mode], variant)
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, kernel {kern_ms:.2f} ms, "
-        f"{value:.3e} bp.s/s, {achieved:.0f} GB/s algorithmic")
+        f"{value:.3e} bp.s/s; VALU issue {roof['frac']}, HBM {roof['hbm_frac']}")
     finite = bool(torch.isfinite(tod).all())
 
     cpu = None
@@ -224,7 +278,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPE,
         "data": "synthetic",
         "config": {
             "workload": f"single detector, {args.mode}, I/Q/U with spin-2: nside {args.nside} map, "
@@ -234,51 +288,48 @@ def main():
             "beam_pixels": S,
             "samples": B,
             "mode": args.mode,
-            "kernel_variant": args.variant,
+            "kernel_variant": variant,
+            "variant_choice": variant_note,
             "map": "seeded Gaussian white noise (values do not affect the kernel's work)",
             "output_finite": finite,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.mode, args.nside, S, B, args.variant),
-            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
 def bench_batch(args, dev):
-    """Many detectors in one persistent launch: autotune (variant, chunk) on a
-    reduced copy, capture zero + launch in a hipGraph, time graph replays."""
+    """Many detectors in one persistent launch per Stokes component (each
+    detector's I, Q and U beams are separate one-field entries): autotune
+    (variant, chunk) on a reduced copy, capture zero + the three launches in
+    one hipGraph, time graph replays."""
     import torch
 
     from tod_generation_mapbased_beam_amd import autotune, inputs
-    from tod_generation_mapbased_beam_amd.batch import BatchPlan, Detector
+    from tod_generation_mapbased_beam_amd.batch import BatchPlan, Detector, capture_plans
     from tod_generation_mapbased_beam_amd.device import DeviceMap
-    from tod_generation_mapbased_beam_amd.workloads import batch_detectors
+    from tod_generation_mapbased_beam_amd.workloads import batch_detectors, batch_plans
 
     t0 = time.time()
     dets, desc = batch_detectors(args.detectors, log=log)
     mp = inputs.white_map(args.nside, seed=0, fields=3, device=dev)
-    dmap = DeviceMap(mp, 1, 2)
-    log(f"batch inputs: {len(dets)} detectors, {sum(desc['B'])} samples, {desc['bps']:.3e} bp.s "
-        f"({time.time() - t0:.1f}s)")
+    maps = [DeviceMap(mp[k:k + 1], -1, -1) for k in range(3)]
+    log(f"batch inputs: {len(dets)} detectors x I/Q/U entries, {sum(desc['B'])} samples, "
+        f"{desc['bps']:.3e} bp.s, tail K {desc['tail_clusters']} ({time.time() - t0:.1f}s)")
 
     def small_plan(var, ch):
-        return BatchPlan([Detector(v, b, tuple(p[:50_000] for p in ps)) for v, b, ps in dets[:8]],
-                         dmap, "bilinear", ch, var)
+        return BatchPlan([Detector(d["entries"][0][0], d["entries"][0][1],
+                                   tuple(p[:50_000] for p in d["params"])) for d in dets[:8]],
+                         maps[0], "bilinear", ch, var)
 
     (variant, chunk), tune_ms = autotune.tune_batch(small_plan, persist=False)
     log(f"autotune: variant {variant}, chunk {chunk} ({tune_ms})")
-    plan = BatchPlan([Detector(v, b, ps) for v, b, ps in dets], dmap, "bilinear", chunk, variant)
-    plan.launch()
+    plans = batch_plans(dets, maps, chunk, variant)
+    for p in plans:
+        p.launch()
     torch.cuda.synchronize()
-    graph = plan.capture()
+    graph = capture_plans(plans)
     for i in range(args.warmup):
         graph.replay()
         torch.cuda.synchronize()
@@ -297,16 +348,19 @@ def bench_batch(args, dev):
     ms_per_step = wall * 1e3 / args.steps
     work = desc["bps"]
     value = work / (ms_per_step / 1e3)
-    achieved = work * BYTES_PER_BPS["bilinear"] / (kern_ms / 1e3) / 1e9
-    finite = all(bool(torch.isfinite(t).all()) for t in plan.tods)
-    log(f"timed {args.steps} replays: {ms_per_step:.1f} ms/step, {value:.3e} bp.s/s, {achieved:.0f} GB/s algorithmic")
+    finite = all(bool(torch.isfinite(t).all()) for p in plans for t in p.tods)
+    roof = roofline(f"batch {args.detectors} detectors seed 0 nside {args.nside}", kern_ms, work, 16, variant)
+    log(f"timed {args.steps} replays: {ms_per_step:.1f} ms/step, {value:.3e} bp.s/s; "
+        f"VALU issue {roof['frac']}, HBM {roof['hbm_frac']}")
     cpu = None
     if not args.no_cpu:
-        v0, b0, p0 = dets[0]
-        cpu = cpu_baseline(v0, b0, p0, mp.cpu().numpy(), args.nside, desc["B"][0], desc["S"][0],
+        d0 = dets[0]
+        v0, b0 = d0["entries"][0]
+        cpu = cpu_baseline(v0, b0, d0["params"], mp.cpu().numpy(), args.nside, d0["B"], int(b0.shape[0]),
                            args.cpu_seconds)
-        cpu["sample"] = "detector 0: " + cpu["sample"]
+        cpu["sample"] = "detector 0, I entry on the I/Q/U map: " + cpu["sample"]
         log(f"cpu baseline: {cpu['value']:.3e} bp.s/s on {cpu['cores']} cores")
+    S_all = [s for ss in desc["S"] for s in ss]
     out = {
         "metric": METRIC,
         "value": value,
@@ -318,30 +372,26 @@ def bench_batch(args, dev):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPE,
         "data": "synthetic",
         "config": {
-            "workload": f"{len(dets)} detectors in one persistent hipGraph-captured launch: elliptical Gaussian "
-                        "beams (FWHM 25-40', e 0-0.2) at nside_beam 4096, cuts 0.97-0.999, last 3% of power "
-                        "k-means clustered, focal-plane offsets <= 2 deg, scan chunks 2e5-2e6 samples at 20 Hz, "
-                        f"nside {args.nside} I/Q/U map with spin-2",
+            "workload": f"{len(dets)} detectors, each with separate I, Q and U beam entries (elliptical Gaussian "
+                        "FWHM 25-40', e 0-0.2; Q/U with 1% cross-polar leakage; no spin-2 for Q-only / U-only "
+                        "entries, as the reference) at nside_beam 4096, cuts 0.97-0.999, last 3% of power "
+                        f"k-means clustered (K = {desc['tail_clusters']} from the clustering calibration), 2-D "
+                        "focal-plane offsets <= 2 deg, scan chunks 2e5-2e6 samples at 20 Hz; one persistent "
+                        f"launch per Stokes map (nside {args.nside}), the three captured in one hipGraph",
             "detectors": len(dets),
+            "entries": 3 * len(dets),
             "samples_total": int(sum(desc["B"])),
-            "beam_pixels_range": [int(min(desc["S"])), int(max(desc["S"]))],
+            "beam_pixels_range": [int(min(S_all)), int(max(S_all))],
             "bps_per_step": work,
             "kernel_variant": int(variant),
             "chunk": int(chunk),
             "map": "seeded Gaussian white noise (values do not affect the kernel's work)",
             "output_finite": finite,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -111,3 +111,19 @@ class BatchPlan:
     def failed_items(self):
         """Indices of items whose pointings held non-finite values."""
         return np.nonzero(self.status.cpu().numpy()[: self.n_items])[0].tolist()
+
+
+def capture_plans(plans):
+    """Record zero + launch of several plans (e.g. one per Stokes component)
+    into one HIP graph, in order; returns the torch graph.  Launch each plan
+    once first so that one-time setup happens outside the capture."""
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            for p in plans:
+                p.zero()
+                p.launch()
+    torch.cuda.current_stream().wait_stream(s)
+    return g

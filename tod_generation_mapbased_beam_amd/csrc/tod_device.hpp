@@ -2319,8 +2319,13 @@ __device__ __forceinline__ bool batch_next(const BatchArgs& ba, BatchCursor& c, 
             if (tid == 0 && ba.status) atomicOr(&ba.status[it], 1);
             continue;
         }
-        if (!c.have_frame) {  // beam frame from the item's first finite sample seen here
-            const BeamFrame f = fp_beam_frame(beam_src(a), b, fp_scratch(), margin_px);
+        if (!c.have_frame) {
+            // beam frame from the item's first finite sample, whichever
+            // workgroup computes it: every workgroup lays out the item's
+            // samples the same way, so results do not depend on the claims
+            int64_t b0 = 0;
+            while (b0 < b && !finite_sample(a, b0)) ++b0;
+            const BeamFrame f = fp_beam_frame(beam_src(a), b0, fp_scratch(), margin_px);
             bf = f;
             c.have_frame = true;
         }

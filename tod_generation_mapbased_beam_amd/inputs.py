@@ -184,12 +184,12 @@ def select_beam(values, vecs, power_cut: float):
     return np.ascontiguousarray(vecs[sel].astype(np.float32)), bv, sel
 
 
-def gaussian_beam(nside_beam: int, fwhm_arcmin: float, power_cut: float = 0.99,
-                  ellipticity: float = 0.0, angle: float = 0.0, extent_sigma: float = 6.0):
-    """Gaussian beam as a HEALPix map centred on the north pole, cut and
-    normalised.  ``ellipticity`` e = 1 - sigma_minor/sigma_major with the
-    geometric-mean width fixed by ``fwhm_arcmin``; ``angle`` orients the
-    major axis in the tangent plane.  Returns (vec_orig, beam_vals, values)."""
+def gaussian_beam_map(nside_beam: int, fwhm_arcmin: float, ellipticity: float = 0.0,
+                      angle: float = 0.0, extent_sigma: float = 6.0):
+    """Uncut Gaussian beam on the HEALPix pixels of a cap around the north
+    pole: (values, unit vectors, colatitudes, longitudes).  ``ellipticity``
+    e = 1 - sigma_minor/sigma_major with the geometric-mean width fixed by
+    ``fwhm_arcmin``; ``angle`` orients the major axis in the tangent plane."""
     sigma = fwhm_arcmin * ARCMIN / math.sqrt(8 * math.log(2))
     q = 1.0 - ellipticity
     s_maj = sigma / math.sqrt(q)
@@ -199,7 +199,15 @@ def gaussian_beam(nside_beam: int, fwhm_arcmin: float, power_cut: float = 0.99,
     x = th * np.cos(ph - angle)
     y = th * np.sin(ph - angle)
     vals = np.exp(-0.5 * ((x / s_maj) ** 2 + (y / s_min) ** 2))
-    vecs = hpx.pix2vec(nside_beam, pix)
+    return vals, hpx.pix2vec(nside_beam, pix), th, ph
+
+
+def gaussian_beam(nside_beam: int, fwhm_arcmin: float, power_cut: float = 0.99,
+                  ellipticity: float = 0.0, angle: float = 0.0, extent_sigma: float = 6.0):
+    """Gaussian beam as a HEALPix map centred on the north pole, cut and
+    normalised (gaussian_beam_map, select_beam).  Returns (vec_orig,
+    beam_vals, values)."""
+    vals, vecs, _, _ = gaussian_beam_map(nside_beam, fwhm_arcmin, ellipticity, angle, extent_sigma)
     vec_orig, bv, sel = select_beam(vals, vecs, power_cut)
     return vec_orig, bv, vals[sel]
 
