@@ -151,6 +151,18 @@ def cpu_baseline(vec, bv, params, mp_host, nside, B, S, target_s):
     }
 
 
+def copy_calibration(dev):
+    """A 1 GiB device-to-device copy (1 GiB read + 1 GiB write of known size):
+    the reference point that scales FETCH_SIZE / WRITE_SIZE to bytes."""
+    import torch
+
+    src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +188,9 @@ def main():
                     help="single: one detector, one day (default); batch: 64 detectors with uneven "
                          "beams and scan chunks in one persistent, graph-captured launch")
     ap.add_argument("--detectors", type=int, default=64)
+    ap.add_argument("--batch-chunk", type=int, default=0,
+                    help="batch: samples per claim; 0 (default) autotunes (build, chunk), otherwise "
+                         "the given chunk with --variant (0 when -1)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -234,11 +249,7 @@ def main():
             ev1.record()
 
     if args.copy_calib:
-        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
-        dst = torch.empty_like(src)
-        dst.copy_(src)  # 1 GiB read + 1 GiB write of known size
-        torch.cuda.synchronize()
-        del src, dst
+        copy_calibration(dev)
     for i in range(args.warmup):
         step()
         torch.cuda.synchronize()
@@ -323,9 +334,15 @@ def bench_batch(args, dev):
                                    tuple(p[:50_000] for p in d["params"])) for d in dets[:8]],
                          maps[0], "bilinear", ch, var)
 
-    (variant, chunk), tune_ms = autotune.tune_batch(small_plan, persist=False)
-    log(f"autotune: variant {variant}, chunk {chunk} ({tune_ms})")
+    if args.batch_chunk > 0:
+        variant, chunk = max(args.variant, 0), args.batch_chunk
+        log(f"given: variant {variant}, chunk {chunk}")
+    else:
+        (variant, chunk), tune_ms = autotune.tune_batch(small_plan, persist=False)
+        log(f"autotune: variant {variant}, chunk {chunk} ({tune_ms})")
     plans = batch_plans(dets, maps, chunk, variant)
+    if args.copy_calib:
+        copy_calibration(dev)
     for p in plans:
         p.launch()
     torch.cuda.synchronize()

@@ -2,12 +2,15 @@
 
 The GPU counterpart of the reference's empirical runtime calibration
 (tod_calibrate.py:198-364, which sweeps processes x threads x batch size and
-writes the winner back to the config): here the knobs are the kernel variant
-(footprint kernel compiled for 2 or 4 waves per SIMD, or the direct kernel
-with 1/2/4 samples per wavefront) and, for batched launches, the number of
-samples per work-queue claim.  Candidates are timed with HIP events on a
-bounded prefix of the real workload; the winner is stored in a JSON cache
-keyed by device and workload shape and reused by later runs.
+writes the winner back to the config, sample_based_tod_generation_gridint.py:
+339-362): here the knobs are the kernel's launch and tile shape -- the
+footprint kernel compiled for 2 or 4 waves per SIMD, its tile variants
+(beam pixels per phase-1 claim 128/256/512 x 64-cell chunks per lane in the
+footprint sweep 3/4/5), or the direct kernel -- and, for batched launches,
+the build and the number of samples per work-queue claim.  Candidates are
+timed with HIP events on a bounded prefix of the real workload; the winner is
+stored in a JSON cache keyed by device and workload shape and picked up by
+beam_tod_batch and bench.py.
 """
 
 from __future__ import annotations
@@ -20,6 +23,10 @@ import torch
 
 CACHE_ENV = "TODG_TUNE_CACHE"
 SINGLE_VARIANTS = (0, 8, 2)
+# footprint-kernel tile variants (include/todgpu.h): 16 + 3 i + j, claims of
+# {128, 256, 512}[i] beam pixels, {3, 4, 5}[j] sweep chunks; built for I/Q/U
+# with spin-2 and for one field
+TILE_VARIANTS = tuple(range(16, 25))
 BATCH_VARIANTS = (0, 8)
 BATCH_CHUNKS = (1, 4, 16, 64)
 
@@ -87,13 +94,26 @@ def _time_ms(fn, reps: int) -> float:
     return ts[len(ts) // 2]
 
 
+def single_candidates(mode: str, dmap):
+    """Variants worth timing for a workload: the 2/4-waves builds and the
+    direct kernel, plus the tile variants where they are built."""
+    qu = dmap.c_q >= 0 and dmap.c_u >= 0
+    cand = list(SINGLE_VARIANTS)
+    if mode in ("bilinear", "nearest") and ((qu and dmap.cs == 4) or (not qu and dmap.cs == 1)):
+        cand += [v for v in TILE_VARIANTS if v != 20]  # (20 is variant 0's tile)
+    return tuple(cand)
+
+
 def tune_single(vec_orig, beam_vals, params, dmap, mode: str = "bilinear",
-                variants=SINGLE_VARIANTS, max_samples: int = 16384, reps: int = 3,
+                variants=None, max_samples: int = 16384, reps: int = 3,
                 persist: bool = True):
     """Time each kernel variant on the first ``max_samples`` samples of a
     single-detector workload; returns (best_variant, {variant: ms})."""
     from . import kernels
     from .device import to_dev
+
+    if variants is None:
+        variants = single_candidates(mode, dmap)
 
     f32 = torch.float32
     B = min(int(params[1].shape[0]), int(max_samples))
