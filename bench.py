@@ -43,8 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one wave64 VALU
 # instruction per SIMD every 4 cycles (16 lanes; binary32 and binary64 alike)
 VALU_PEAK_GINST = 1024 * 2.4 / 4.0
-# neighbours x 3 fields x float32 (learned: the corrector's 4 x 4 neighbourhood)
-BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 192}
+# neighbours x 3 fields x float32 (learned: both channels of the factorised
+# corrector; learned-mlp: the MLP corrector's 4 x 4 neighbourhood)
+BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 96, "learned-mlp": 192}
 DTYPE = "mixed: f64 rotation, index path and accumulation; f32 map, weights and near-boresight spin-2"
 
 
@@ -177,9 +178,15 @@ def main():
     ap.add_argument("--fwhm", type=float, default=30.0)
     ap.add_argument("--samples", type=int, default=1728000)
     ap.add_argument("--fs", type=float, default=20.0)
-    ap.add_argument("--mode", choices=["bilinear", "nearest", "learned"], default="bilinear",
-                    help="learned: the fused corrector kernel with seeded random parameters "
-                         "(its work does not depend on their values), bound 4")
+    ap.add_argument("--mode", choices=["bilinear", "nearest", "learned", "learned-mlp"], default="bilinear",
+                    help="learned: the factorised corrector with the package's fitted coefficients (its "
+                         "channel maps are built before timing, like the packed map; the line adds the "
+                         "plain kernel's time on the same workload and the residual against the bicubic "
+                         "judge on a synthetic sky); learned-mlp: the per-beam-pixel MLP corrector with "
+                         "seeded random parameters (its work does not depend on their values), bound 4")
+    ap.add_argument("--judge-samples", type=int, default=100_000,
+                    help="learned: samples of the day (evenly spaced) on which the residual against the "
+                         "bicubic judge is measured on a synthetic nside-2048 sky; 0 skips it")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
@@ -220,7 +227,7 @@ def main():
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
     variant, variant_note = args.variant, "given"
     if variant < 0:
-        if args.mode == "learned":
+        if args.mode in ("learned", "learned-mlp"):
             variant, variant_note = 0, "the corrector has one build"
         else:
             from tod_generation_mapbased_beam_amd import autotune
@@ -230,7 +237,21 @@ def main():
             variant_note = "autotuned on the first 131072 samples: " + ", ".join(
                 f"{k}: {v:.2f} ms" for k, v in sorted(tms.items()))
             log(f"autotune: variant {variant} ({variant_note})")
+    prep_ms = None
     if args.mode == "learned":
+        from tod_generation_mapbased_beam_amd import fcorr
+
+        fc = fcorr.FactorisedCorrector.load_default()
+        torch.cuda.synchronize()
+        t_prep = time.perf_counter()
+        fmaps = fc.maps(mp, 1, 2)
+        torch.cuda.synchronize()
+        prep_ms = (time.perf_counter() - t_prep) * 1e3
+        log(f"channel maps built in {prep_ms:.1f} ms")
+
+        def fn(v, ax, ca, sa, bp, cp, sp, nside, _dmap, b, nb, ns, t, c_q, c_u, variant=0):
+            fcorr.gather_accum_fcorr(v, ax, ca, sa, bp, cp, sp, nside, fmaps, b, nb, ns, t)
+    elif args.mode == "learned-mlp":
         from tod_generation_mapbased_beam_amd import corrector
 
         corr = torch.as_tensor(corrector.init_params(0) * 2.0, device=dev)
@@ -272,8 +293,12 @@ def main():
         f"{value:.3e} bp.s/s; VALU issue {roof['frac']}, HBM {roof['hbm_frac']}")
     finite = bool(torch.isfinite(tod).all())
 
+    learned = None
+    if args.mode == "learned":
+        learned = learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms,
+                                 prep_ms, B, S)
     cpu = None
-    if not args.no_cpu and args.mode != "learned":
+    if not args.no_cpu and args.mode in ("bilinear", "nearest"):
         mp_host = mp.cpu().numpy()
         cpu = cpu_baseline(vec, bv, params, mp_host, args.nside, B, S, args.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.3e} bp.s/s on {cpu['cores']} cores")
@@ -307,7 +332,58 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if learned is not None:
+        out["learned"] = learned
     print(json.dumps(out), flush=True)
+
+
+def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms, prep_ms, B, S):
+    """The factorised corrector's line: the plain bilinear kernel timed on the
+    same workload (the autotuner's variant), the time ratio with and without
+    the one-off channel build, and the RMS / max residual against the bicubic
+    judge on a synthetic sky, relative to bilinear's."""
+    import torch
+
+    from tod_generation_mapbased_beam_amd import autotune, fcorr, inputs, kernels
+
+    pv, _ = autotune.tune_single(dvec, dbv, dparams, dmap, mode="bilinear", max_samples=131072, reps=3)
+    tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
+    ts = []
+    for i in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        kernels._gather_accum_fused_jit(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, variant=pv)
+        e1.record()
+        torch.cuda.synchronize()
+        if i:
+            ts.append(e0.elapsed_time(e1))
+    plain_ms = float(np.mean(ts))
+    out = {"method": "factorised corrector (todg_fcorr_channels + todg_gather_accum_fcorr)",
+           "plain_bilinear_kernel_ms": plain_ms, "plain_variant": int(pv), "kernel_ms": kern_ms,
+           "time_ratio_over_plain": kern_ms / plain_ms, "channel_build_ms": prep_ms,
+           "time_ratio_with_channel_build": (kern_ms + prep_ms) / plain_ms,
+           "coefficients": os.path.relpath(fcorr.DEFAULT_PATH, REPO)}
+    log(f"learned: kernel {kern_ms:.2f} ms vs plain {plain_ms:.2f} ms (variant {pv}): "
+        f"{kern_ms / plain_ms:.3f}x; with the channel build {(kern_ms + prep_ms) / plain_ms:.3f}x")
+    if args.judge_samples > 0:
+        t0 = time.time()
+        sky = torch.as_tensor(inputs.synfast_iqu(args.nside, seed=7, device=dev), device=dev,
+                              dtype=torch.float32).contiguous()
+        log(f"judge sky nside {args.nside} ({time.time() - t0:.1f}s)")
+        step = max(1, B // args.judge_samples)
+        idx = np.arange(0, B, step)[: args.judge_samples]
+        sub = [torch.as_tensor(np.ascontiguousarray(np.asarray(p)[idx]), device=dev) for p in params]
+        nb = idx.size
+        sdm = kernels.device_map_for(sky, 1, 2)
+        tb, tc, tf = (torch.zeros((3, nb), dtype=torch.float64, device=dev) for _ in range(3))
+        kernels._gather_accum_fused_jit(dvec, *sub, args.nside, sdm, dbv, nb, S, tb, 1, 2)
+        kernels._gather_accum_bicubic_jit(dvec, *sub, args.nside, sdm, dbv, nb, S, tc, 1, 2)
+        fcorr.gather_accum_fcorr(dvec, *sub, args.nside, fc.maps(sky, 1, 2), dbv, nb, S, tf)
+        j = fcorr.judge(tf, tb, tc)
+        out["judge"] = {"samples": int(nb), "sky": "synthetic I/Q/U, spectrum of inputs.cl_spectrum, seed 7",
+                        "rows": "tod rows 0, 1, 2 = I, Q, U", **j}
+        log(f"learned: RMS residual vs bicubic / bilinear's = {j['rms_ratio']}, max {j['max_ratio']}")
+    return out
 
 
 def bench_batch(args, dev):

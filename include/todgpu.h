@@ -283,6 +283,35 @@ int todg_gather_accum_learned(const float *vec_orig, const float *axes, const fl
                               int c_q, int c_u, const float *params, float bound,
                               int64_t n_corrected, unsigned long long *stats, void *stream);
 
+/* Factorised corrector, map side: the two stencil channels of one map field
+ * (field: npix float32, RING order).  Per pixel p on ring r, the 16 features
+ * are F_0 = m(p) and, for (dr, ds) in {0..3}^2 minus (0, 0) in row-major
+ * order, the mean of the values at rings r -+ dr, longitudes phi_p -+ ds
+ * dphi_r (each linearly interpolated between the two ring pixels around it)
+ * minus m(p); features with dr > 0 are 0 on the 3 rings next to each pole.
+ * out0[p * stride] = sum_j coef[j] F_j and out1[p * stride] = sum_j coef[16 +
+ * j] F_j (coef: HOST pointer to 32 doubles; out0 / out1 device float32, e.g.
+ * one slot of a packed map: todg_map_slots).  No reference counterpart: the
+ * corrector is this library's own (the learned interpolation corrector's
+ * fast form). */
+int todg_fcorr_channels(int64_t nside, const float *field, const double *coef, float *out0,
+                        float *out1, int64_t stride, void *stream);
+
+/* Factorised corrector, TOD side: arguments of todg_gather_accum_fused with
+ * two packed maps of the same slot layout -- map_packed holding the main
+ * channel A0 and map1_packed the correction channel A1 (both from
+ * todg_fcorr_channels) -- and the status word of todg_gather_accum.  Each
+ * beam pixel's bilinear neighbours p_k and weights w_k (the reference's,
+ * numba_healpy.py:358-499) give bv sum_k w_k (A0(p_k) + (1 - g) g A1(p_k)), g
+ * = w_2 + w_3 the lower ring's weight; Q/U are rotated per pixel by the
+ * reference's spin-2 angle (tod_bilinear.py:403-432) after both channels are
+ * combined.  With A0 = map and A1 = 0 this is the plain bilinear TOD. */
+int todg_gather_accum_fcorr(const float *vec_orig, const float *axes, const float *cos_a,
+                            const float *sin_a, const float *ax_pts, const float *cos_p,
+                            const float *sin_p, int64_t nside, const float *map_packed,
+                            const float *map1_packed, int C, const float *beam_vals, int64_t B,
+                            int64_t S, double *tod, int c_q, int c_u, int *status, void *stream);
+
 /* C (M x N) = alpha op(A) op(B) + beta C on BF16 MFMA (row-major float32
  * operands rounded to bfloat16; ta / tb transpose A / B). */
 int todg_corr_gemm_bf16(int M, int N, int K, const float *A, int lda, int ta, const float *B,

@@ -49,6 +49,11 @@ _SIGS = {
     "or_gather_accum_bicubic": (
         None, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, ctypes.c_int]),
     "or_interp_values": (None, [_i64, _p, _i64, _i64, _p, _p, ctypes.c_int, _p]),
+    "or_pixel_ring": (_i64, [_i64, _i64]),
+    "or_fcorr_features": (None, [_i64, _p, _i64, _p]),
+    "or_fcorr_channels": (None, [_i64, _p, _p, _p, _p]),
+    "or_gather_accum_fcorr": (
+        None, [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p, _i64, _p, _i64, _i64, _p, _i64, _i64, ctypes.c_int]),
 }
 
 
@@ -282,3 +287,45 @@ def interp_values(nside, mp, z, phi, mode):
 
 def max_threads():
     return int(lib().or_max_threads())
+
+
+# ------------------------------------------- factorised corrector (own)
+
+def pixel_ring(nside, p):
+    """RING ring index (1 .. 4 nside - 1) of pixel p."""
+    return int(lib().or_pixel_ring(int(nside), int(p)))
+
+
+def fcorr_features(nside, field, p):
+    """The factorised corrector's 16 stencil features of pixel p (the
+    library's own definition; no reference counterpart)."""
+    m = _c(field, np.float32)
+    F = np.zeros(16)
+    lib().or_fcorr_features(int(nside), _ptr(m), int(p), _ptr(F))
+    return F
+
+
+def fcorr_channels(nside, field, coef):
+    """(A0, A1) float64 maps of one field for the 32 coefficients."""
+    m = _c(field, np.float32)
+    c = _c(coef, np.float64)
+    o0 = np.zeros(m.shape[0])
+    o1 = np.zeros(m.shape[0])
+    lib().or_fcorr_channels(int(nside), _ptr(m), _ptr(c), _ptr(o0), _ptr(o1))
+    return o0, o1
+
+
+def gather_accum_fcorr(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, A0, A1, beam_vals,
+                       B, S, tod, c_q=-1, c_u=-1, nthreads=0):
+    """Factorised-corrector TOD (accumulates into tod (C, B) float64)."""
+    a0 = _c(A0, np.float32)
+    a1 = _c(A1, np.float32)
+    if a0.ndim == 1:
+        a0 = a0[None]
+        a1 = a1[None]
+    ops = [_c(x, np.float32) for x in (vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, beam_vals)]
+    t = np.ascontiguousarray(tod, dtype=np.float64)
+    lib().or_gather_accum_fcorr(*[_ptr(o) for o in ops[:7]], int(nside), _ptr(a0), _ptr(a1), a0.shape[0],
+                                _ptr(ops[7]), int(B), int(S), _ptr(t), int(c_q), int(c_u), int(nthreads))
+    tod[...] = t
+    return tod

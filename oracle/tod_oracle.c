@@ -984,3 +984,157 @@ void or_interp_values(int64_t nside, const float *mp, int64_t npix, int64_t N, c
         out[i] = acc;
     }
 }
+
+/* ------------------------------------------ factorised corrector (own) */
+
+/* The factorised corrector is this library's own construction (the fast
+ * form of its learned interpolation corrector; the reference has no
+ * counterpart).  These restate the definitions the library documents for
+ * todg_fcorr_channels and todg_gather_accum_fcorr, independently of the GPU
+ * code: ring of a pixel by exact integer square roots, ring values by
+ * division instead of the GPU's reciprocal multiplication, per-neighbour
+ * combination instead of the GPU's per-pixel weight sums. */
+
+static int64_t fc_isqrt(int64_t x)
+{
+    int64_t s = (int64_t)sqrt((double)x);
+    while (s * s > x)
+        --s;
+    while ((s + 1) * (s + 1) <= x)
+        ++s;
+    return s;
+}
+
+/* RING scheme: ring (1 .. 4 nside - 1) of pixel p. */
+int64_t or_pixel_ring(int64_t nside, int64_t p)
+{
+    int64_t npix = 12 * nside * nside, ncap = 2 * nside * (nside - 1);
+    if (p < ncap)
+        return (1 + fc_isqrt(1 + 2 * p)) / 2;
+    if (p < npix - ncap)
+        return (p - ncap) / (4 * nside) + nside;
+    return 4 * nside - (1 + fc_isqrt(1 + 2 * (npix - 1 - p))) / 2;
+}
+
+static double fc_ring_value(int64_t nside, const float *m, int64_t r, double phi)
+{
+    int64_t n, fp;
+    double phi0, dphi;
+    or_ring_info(nside, r, 12 * nside * nside, &n, &fp, &phi0, &dphi);
+    double t = (phi - phi0) / dphi;
+    double jf = floor(t);
+    double f = t - jf;
+    int64_t j = ((int64_t)jf % n + n) % n;
+    int64_t j1 = (j + 1) % n;
+    return (1.0 - f) * (double)m[fp + j] + f * (double)m[fp + j1];
+}
+
+/* The 16 stencil features of pixel p (F[0] = m(p); F[1..15] for (dr, ds) in
+ * {0..3}^2 minus (0, 0), row-major). */
+void or_fcorr_features(int64_t nside, const float *m, int64_t p, double *F)
+{
+    int64_t r = or_pixel_ring(nside, p);
+    int64_t n, fp;
+    double phi0, dphi;
+    or_ring_info(nside, r, 12 * nside * nside, &n, &fp, &phi0, &dphi);
+    double phi = phi0 + (double)(p - fp) * dphi;
+    double mp = (double)m[p];
+    int ok = r >= 4 && r <= 4 * nside - 4;
+    int j = 0;
+    F[j++] = mp;
+    for (int dr = 0; dr < 4; ++dr) {
+        for (int ds = 0; ds < 4; ++ds) {
+            if (dr == 0 && ds == 0)
+                continue;
+            double v = 0.0;
+            double pl = phi - ds * dphi, ph = phi + ds * dphi;
+            if (dr == 0) {
+                v = 0.5 * (fc_ring_value(nside, m, r, pl) + fc_ring_value(nside, m, r, ph)) - mp;
+            } else if (ok) {
+                if (ds == 0)
+                    v = 0.5 * (fc_ring_value(nside, m, r - dr, phi) + fc_ring_value(nside, m, r + dr, phi)) - mp;
+                else
+                    v = 0.25 * (fc_ring_value(nside, m, r - dr, pl) + fc_ring_value(nside, m, r - dr, ph) +
+                                fc_ring_value(nside, m, r + dr, pl) + fc_ring_value(nside, m, r + dr, ph)) -
+                        mp;
+            }
+            F[j++] = v;
+        }
+    }
+}
+
+/* out0[p] = sum_j coef[j] F_j(p), out1[p] = sum_j coef[16 + j] F_j(p). */
+void or_fcorr_channels(int64_t nside, const float *m, const double *coef, double *out0, double *out1)
+{
+    int64_t npix = 12 * nside * nside;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < npix; ++p) {
+        double F[16];
+        or_fcorr_features(nside, m, p, F);
+        double s0 = 0.0, s1 = 0.0;
+        for (int j = 0; j < 16; ++j) {
+            s0 += coef[j] * F[j];
+            s1 += coef[16 + j] * F[j];
+        }
+        out0[p] = s0;
+        out1[p] = s1;
+    }
+}
+
+/* Corrected beam TOD: or_gather_accum_fused with each neighbour's value
+ * A0(p_k) + (1 - g) g A1(p_k), g = w_2 + w_3, per field (A0, A1: (C, npix)
+ * field-major float32), spin-2 applied to the combined Q/U values. */
+void or_gather_accum_fcorr(const float *vec_orig, const float *axes, const float *cos_a,
+                           const float *sin_a, const float *ax_pts, const float *cos_p,
+                           const float *sin_p, int64_t nside, const float *A0, const float *A1,
+                           int64_t C, const float *beam_vals, int64_t B, int64_t S, double *tod,
+                           int64_t c_q, int64_t c_u, int nthreads)
+{
+    int has_qu = (c_q >= 0 && c_u >= 0);
+    int64_t npix_total = 12 * nside * nside;
+#ifdef _OPENMP
+    if (nthreads > 0)
+        omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t b = 0; b < B; ++b) {
+        rot_params r = load_rot(axes, cos_a, sin_a, ax_pts, cos_p, sin_p, b);
+        double z_pts = py_max(-1.0, py_min(1.0, r.bz));
+        double sth_pts = sqrt(py_max(0.0, 1.0 - r.bz * r.bz));
+        double phi_pts = wrap_atan2(r.by, r.bx);
+        double acc[8];
+        for (int64_t c = 0; c < C && c < 8; ++c)
+            acc[c] = tod[c * B + b];
+        for (int64_t s = 0; s < S; ++s) {
+            double vx, vy, vz;
+            rotate_sample(&r, vec_orig, s, &vx, &vy, &vz);
+            double z = py_max(-1.0, py_min(1.0, vz));
+            double phi_w = wrap_atan2(vy, vx);
+            int64_t p[4];
+            double w[4], zn[4], phin[4];
+            or_ring_interp_with_angles(nside, z, phi_w, npix_total, p, w, zn, phin);
+            double uc = (w[0] + w[1]) * (w[2] + w[3]);
+            double bv = (double)beam_vals[s];
+            for (int k = 0; k < 4; ++k) {
+                double wk = w[k] * bv;
+                if (has_qu) {
+                    double c2, s2;
+                    spin2_neighbour(zn[k], phin[k], z_pts, sth_pts, phi_pts, &c2, &s2);
+                    double q = (double)A0[c_q * npix_total + p[k]] + uc * (double)A1[c_q * npix_total + p[k]];
+                    double u = (double)A0[c_u * npix_total + p[k]] + uc * (double)A1[c_u * npix_total + p[k]];
+                    acc[c_q] += wk * (q * c2 + u * s2);
+                    acc[c_u] += wk * (-q * s2 + u * c2);
+                }
+                for (int64_t c = 0; c < C && c < 8; ++c) {
+                    if (has_qu && (c == c_q || c == c_u))
+                        continue;
+                    acc[c] += wk * ((double)A0[c * npix_total + p[k]] + uc * (double)A1[c * npix_total + p[k]]);
+                }
+            }
+        }
+        for (int64_t c = 0; c < C && c < 8; ++c)
+            tod[c * B + b] = acc[c];
+    }
+}

@@ -25,7 +25,7 @@ LIB_NAME = "libtodgpu.so"
 LIB_PATH = os.path.join(PKG_DIR, LIB_NAME)
 
 SOURCES = ["tod_fp_bilinear.hip", "tod_fp_nearest.hip", "tod_fp_bicubic.hip", "tod_fp_learned.hip",
-           "tod_kernels.hip", "corrector.hip", "beam_tools.hip"]
+           "tod_fp_fcorr.hip", "tod_kernels.hip", "corrector.hip", "beam_tools.hip"]
 HEADERS = ["healpix_ring.hpp", "internal.hpp", "corrector.hpp", "tod_device.hpp"]
 
 HIPCC_FLAGS = [

@@ -52,6 +52,10 @@ struct TodArgs {
     int64_t corr_count;                // beam pixels [0, corr_count) take the corrector, the rest bilinear
     int* status;                       // bit 0 set when a sample had a non-finite operand, or null
     int64_t* dump;                     // index-dump builds only: see FpDump
+    // factorised corrector (footprint mode 4 only): the correction channel,
+    // packed like map; each beam pixel's bilinear weights times its ring
+    // weight product (1 - g) g go with it (FcorrWeights)
+    const float* map1;
 };
 
 // Index dump of the footprint kernel (its DUMP builds): for every (b, s) of
@@ -322,10 +326,11 @@ struct FpGlobals {
     int64_t npix;
     int nside;
     int nslots;
+    const float* map1;  // mode 4's correction channel
 };
 
 __device__ __forceinline__ FpGlobals fp_globals(const TodArgs& a) {
-    return FpGlobals{a.map, a.rings, a.npix, a.nside, a.nslots};
+    return FpGlobals{a.map, a.rings, a.npix, a.nside, a.nslots, a.map1};
 }
 
 // Spin-2 inputs of one sample (uniform).
@@ -384,6 +389,15 @@ constexpr int kFpSmemBytesCorr = kOffBias + (todg_corr::kH + todg_corr::kOut) * 
 static_assert(kOffStage % 16 == 0 && kOffBias % 16 == 0, "16-byte LDS vectors");
 static_assert(kFpSmemBytesCorr <= 160 * 1024, "one workgroup per CU");
 static_assert(kFpSmemBytes <= 80 * 1024, "two workgroups per CU");
+// mode 4 (factorised corrector) splits the weight grid: cells [0,
+// kFpCapFc) hold the main channel's weights, cells [kFpCapFc, kFpCap) the
+// correction channel's, so its footprints are laid out up to kFpCapFc cells
+// (a 30' beam at nside 2048 needs ~2000); it adds the correction weights of
+// the side list
+constexpr int kFpCapFc = kFpCap / 2;
+constexpr int kOffSideW1 = kFpSmemBytes;
+constexpr int kFpSmemBytesFc = kOffSideW1 + 8 * kSideCap;
+static_assert(kFpSmemBytesFc <= 80 * 1024, "two workgroups per CU");
 
 template <typename T>
 __device__ __forceinline__ T& lds_at(unsigned off) {
@@ -441,6 +455,7 @@ __device__ __forceinline__ unsigned char* fp_rtab(int buf) { return fp_smem + kO
 __device__ __forceinline__ double* fp_red() { return &lds_at<double>(kOffRed); }
 __device__ __forceinline__ double* fp_scratch() { return &lds_at<double>(kOffScratch); }
 __device__ __forceinline__ WCell* fp_W() { return &lds_at<WCell>(kOffW); }
+__device__ __forceinline__ WCell* fp_W1() { return &lds_at<WCell>(kOffW + kFpCapFc * (int)sizeof(WCell)); }
 
 // The per-sample operands a stage needs after the sample's layout: the
 // beam, the tod destination, the sample index and phase 1's claim counter
@@ -540,25 +555,66 @@ __device__ __forceinline__ Acc4 fp_direct(const FpGlobals& a, const FpSpin& sp, 
     return r;
 }
 
+// Mode 4's direct contribution of one map pixel: the main channel with
+// weight w plus the correction channel with weight w1, then spin-2 on slots
+// 0/1 when QU.
+template <bool QU, int CS>
+__device__ __forceinline__ Acc4 fp_direct_fc(const FpGlobals& a, const FpSpin& sp, int ring, int pos,
+                                             double w, double w1) {
+    Acc4 r = {{0.0, 0.0, 0.0, 0.0}};
+    const RingEntry& e = a.rings[ring];
+    const float4 m = load_px<CS>(a.map, e.first + pos, a.npix);
+    const float4 m1 = load_px<CS>(a.map1, e.first + pos, a.npix);
+    double v[kMaxSlots] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < CS; ++k) v[k] = fma(w1, (double)slot(m1, k), w * (double)slot(m, k));
+    if constexpr (QU) {
+        double c2d, s2d;
+        fp_spin2(sp, ring, pos, e.z, e.sth, e.phi0 + (double)pos * e.dphi, c2d, s2d);
+        r.v[0] = fma(v[0], c2d, v[1] * s2d);
+        r.v[1] = fma(v[1], c2d, -(v[0] * s2d));
+#pragma unroll
+        for (int k = 2; k < CS; ++k)
+            if (k < a.nslots) r.v[k] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < CS; ++k)
+            if (k < a.nslots) r.v[k] = v[k];
+    }
+    return r;
+}
+
+template <bool QU, int CS, bool FC>
+__device__ __forceinline__ Acc4 fp_direct_any(const FpGlobals& a, const FpSpin& sp, int ring, int pos,
+                                              double w, double w1) {
+    if constexpr (FC)
+        return fp_direct_fc<QU, CS>(a, sp, ring, pos, w, w1);
+    else
+        return fp_direct<QU, CS>(a, sp, ring, pos, w);
+}
+
 __device__ __forceinline__ int* fp_side_count() { return &lds_at<int>(kOffSide); }
 __device__ __forceinline__ unsigned* fp_side_key() { return &lds_at<unsigned>(kOffSideKey); }
 __device__ __forceinline__ double* fp_side_w() { return &lds_at<double>(kOffSideW); }
+__device__ __forceinline__ double* fp_side_w1() { return &lds_at<double>(kOffSideW1); }
 
 // A neighbour pixel (ring, slot pos <= n <= 2^15) outside the laid-out
-// footprint with grid-rounded weight w: appended to the sample's side list,
-// which fp_side_flush sums in pixel order.  When the list is full the pixel
-// is accumulated on the spot (returned), and that sample's last bits then
-// depend on which thread took it.
-template <bool QU, int CS>
+// footprint with grid-rounded weight w (and, in mode 4, correction weight
+// w1): appended to the sample's side list, which fp_side_flush sums in pixel
+// order.  When the list is full the pixel is accumulated on the spot
+// (returned), and that sample's last bits then depend on which thread took
+// it.
+template <bool QU, int CS, bool FC = false>
 __device__ __forceinline__ Acc4 fp_outside(const FpGlobals& a, const FpSpin& sp, int ring, int pos,
-                                           double w) {
+                                           double w, double w1 = 0.0) {
     const int i = atomicAdd(fp_side_count(), 1);
     if (i < kSideCap) {
         fp_side_key()[i] = ((unsigned)ring << 16) | (unsigned)pos;
         fp_side_w()[i] = w;
+        if constexpr (FC) fp_side_w1()[i] = w1;
         return Acc4{{0.0, 0.0, 0.0, 0.0}};
     }
-    return fp_direct<QU, CS>(a, sp, ring, pos, w);
+    return fp_direct_any<QU, CS, FC>(a, sp, ring, pos, w, w1);
 }
 
 __device__ __forceinline__ FpSpin fp_spin_of(const FpFrame* fr);
@@ -566,30 +622,37 @@ __device__ __forceinline__ FpSpin fp_spin_of(const FpFrame* fr);
 // One thread, after phase 2: the side list's pixels in key order (equal keys
 // merged; their grid-rounded weights add exactly), each with its spin-2 and
 // one gather, added to acc in that order; the list is emptied.
-template <bool QU, int CS>
+template <bool QU, int CS, bool FC = false>
 __device__ __noinline__ Acc4 fp_side_flush(FpGlobals a, const FpFrame* fr, Acc4 acc) {
     int* cnt = fp_side_count();
     unsigned* key = fp_side_key();
     double* w = fp_side_w();
+    double* w1 = fp_side_w1();
     const int n = min(*cnt, kSideCap);
     for (int i = 1; i < n; ++i) {  // insertion sort by key
         const unsigned k = key[i];
         const double x = w[i];
+        const double x1 = FC ? w1[i] : 0.0;
         int j = i - 1;
         while (j >= 0 && key[j] > k) {
             key[j + 1] = key[j];
             w[j + 1] = w[j];
+            if constexpr (FC) w1[j + 1] = w1[j];
             --j;
         }
         key[j + 1] = k;
         w[j + 1] = x;
+        if constexpr (FC) w1[j + 1] = x1;
     }
     const FpSpin sp = fp_spin_of(fr);
     for (int i = 0; i < n;) {
         const unsigned k = key[i];
-        double ws = 0.0;
-        for (; i < n && key[i] == k; ++i) ws += w[i];
-        const Acc4 d = fp_direct<QU, CS>(a, sp, (int)(k >> 16), (int)(k & 0xffffu), ws);
+        double ws = 0.0, ws1 = 0.0;
+        for (; i < n && key[i] == k; ++i) {
+            ws += w[i];
+            if constexpr (FC) ws1 += w1[i];
+        }
+        const Acc4 d = fp_direct_any<QU, CS, FC>(a, sp, (int)(k >> 16), (int)(k & 0xffffu), ws, ws1);
         for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
     }
     *cnt = 0;
@@ -658,7 +721,28 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
             for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
         }
     };
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 4) {
+        // bilinear deposits, each also into the correction grid times the
+        // ring weight product (1 - g) g, g = the lower ring's weight
+        Nb o;
+        ring_interp(a.nside, a.rings, fmax(-1.0, fmin(1.0, vz)), phi_w, o);
+        const double uc = (o.w[0] + o.w[1]) * (o.w[2] + o.w[3]);
+        WCell* W1 = fp_W1();
+        for (int k = 0; k < 4; ++k) {
+            const double w = qround(o.w[k] * bv, M);
+            const double w1 = qround(w * uc, M);
+            const int lr = o.ring[k] - ir_lo;
+            const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
+            if (li >= 0) {
+                atomicAdd(&W[li], (WCell)w);
+                atomicAdd(&W1[li], (WCell)w1);
+            } else {
+                const Acc4 d = fr->direct ? fp_direct_fc<QU, CS>(a, sp, o.ring[k], o.pos[k], w, w1)
+                                          : fp_outside<QU, CS, true>(a, sp, o.ring[k], o.pos[k], w, w1);
+                for (int j = 0; j < kMaxSlots; ++j) acc.v[j] += d.v[j];
+            }
+        }
+    } else if constexpr (MODE == 0) {
         Nb o;
         ring_interp(a.nside, a.rings, fmax(-1.0, fmin(1.0, vz)), phi_w, o);
         for (int k = 0; k < 4; ++k) {
@@ -817,7 +901,7 @@ __device__ __forceinline__ void fill_ring_cells(unsigned char* rtab, int k0, int
 }
 
 __device__ __forceinline__ bool fp_layout(const TodArgs& a, int64_t b, const BeamFrame& bf,
-                                          FpFrame* fr, FpRings R, unsigned char* rtab) {
+                                          FpFrame* fr, FpRings R, unsigned char* rtab, int cap) {
     const int lane = threadIdx.x & 63;
     const int nside = a.nside;
     Sample q;
@@ -973,7 +1057,7 @@ __device__ __forceinline__ bool fp_layout(const TodArgs& a, int64_t b, const Bea
             if (lane >= d) incl += t;
         }
         const int off = carry + incl - width;
-        const bool fits = off + width <= kFpCap;
+        const bool fits = off + width <= cap;
         int end = (r < nr && fits) ? off + width : 0;
         if (r < nr) {
             R.i0(r) = i0;
@@ -1111,7 +1195,7 @@ __device__ __forceinline__ int64_t fp_cell_pixel(const FpRings& R, int r, int c)
     return (int64_t)R.first(r) + pos;
 }
 
-template <bool DUMP = false>
+template <bool DUMP = false, bool FC = false>
 __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, int nside,
                                                  int ir_lo, int nr, const FpRings& R, WCell* W,
                                                  double u0, double u1, double u2, float bvf,
@@ -1171,7 +1255,8 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     float pp = fmaf(x2, 105.f / 3456.f, 15.f / 336.f);
     pp = fmaf(x2, pp, 3.f / 40.f);
     pp = fmaf(x2, pp, 1.f / 6.f);
-    const float wb = fmaf(xf * x2, pp, xf) * R.inv_dth_f(la) * bvf;
+    const float gfr = fmaf(xf * x2, pp, xf) * R.inv_dth_f(la);  // lower ring's weight g
+    const float wb = gfr * bvf;
     const float wa = bvf - wb;
     const double wad = (double)wa, wbd = (double)wb;
     // M + round(wa), M + round(wa fa): both in [2^E, 2^(E+1)), so their
@@ -1185,6 +1270,15 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     atomicAdd(ca + 1, (WCell)d1);
     atomicAdd(cb, (WCell)d2);
     atomicAdd(cb + 1, (WCell)d3);
+    if constexpr (FC) {
+        // the correction grid: each deposit times (1 - g) g, grid-rounded
+        const double uc = (double)fmaf(-gfr, gfr, gfr);
+        WCell* W1 = fp_W1();
+        atomicAdd(W1 + ia, (WCell)qround(d0 * uc, M));
+        atomicAdd(W1 + ia + 1, (WCell)qround(d1 * uc, M));
+        atomicAdd(W1 + ib, (WCell)qround(d2 * uc, M));
+        atomicAdd(W1 + ib + 1, (WCell)qround(d3 * uc, M));
+    }
     if constexpr (DUMP) {
         dd.pix[4 * di] = fp_cell_pixel(R, la, ia);
         dd.pix[4 * di + 1] = fp_cell_pixel(R, la, ia + 1);
@@ -1351,9 +1445,9 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
     Acc4 accd = {{0.0, 0.0, 0.0, 0.0}};  // outside-grid sums
     // A sample without a (complete) layout -- its footprint exceeds the grid
     // -- accumulates the neighbours outside the layout directly into
-    // per-thread sums; in nearest and bicubic mode it runs with a fixed beam
-    // pixel per thread, which keeps those sums, and the result, independent
-    // of timing.  (Bilinear keeps the dynamic claims for such samples: the
+    // per-thread sums; in nearest, bicubic and factorised-corrector mode it
+    // runs with a fixed beam pixel per thread, which keeps those sums, and
+    // the result, independent of timing.  (Bilinear keeps the dynamic claims for such samples: the
     // test alone costs its fast loop 3 %, measured; those samples' sums then
     // agree between runs to rounding.)
     if constexpr (MODE != 0) {
@@ -1404,8 +1498,9 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
                 fetch(s1, 0);
             if (s < S) {
                 bool ok = false;
-                if constexpr (MODE == 0)
-                    ok = fp_fast_bilinear<DUMP>(m, nside, ir_lo, nr, R, W, v0, v1, v2, bvf, M, dd, dbase + s);
+                if constexpr (MODE == 0 || MODE == 4)
+                    ok = fp_fast_bilinear<DUMP, MODE == 4>(m, nside, ir_lo, nr, R, W, v0, v1, v2, bvf, M, dd,
+                                                           dbase + s);
                 if constexpr (MODE == 1)
                     ok = fp_fast_nearest<DUMP>(m, nside, ir_lo, nr, R, W, v0, v1, v2, (double)bvf, M, dd,
                                                dbase + s);
@@ -1968,7 +2063,7 @@ constexpr float kSpinF32St2 = 3e-4f;
 // Returns the per-thread sums (outside-grid sums included).
 constexpr int kP2Cells = 4;
 
-template <bool QU, int CS, int P2 = kP2Cells>
+template <bool QU, int CS, int P2 = kP2Cells, bool FC = false>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
                                           const unsigned char* rtab, WCell* W, Acc4 acc) {
     constexpr int kP2 = P2;
@@ -1978,22 +2073,27 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
     const int coinc = __builtin_amdgcn_readfirstlane(fr->coinc_cell);
     const FpSpin sp = fp_spin_of(fr);
     for (int base = wave * 64; base < total; base += kFpThreads * kP2) {
-        double wsum[kP2];
+        double wsum[kP2], w1sum[kP2];
         int rr[kP2], kk[kP2];
-        float4 mv[kP2];
+        float4 mv[kP2], mv1[kP2];
+        WCell* W1 = fp_W1();
 #pragma unroll
         for (int u = 0; u < kP2; ++u) {
             const int j = base + u * kFpThreads + lane;
             const bool in = j < total;
             wsum[u] = in ? (double)W[j] : 0.0;
+            w1sum[u] = (FC && in) ? (double)W1[j] : 0.0;
             rr[u] = in ? (int)rtab[j] : 0;
         }
+        // (mode 4: a cell is live when either grid holds weight)
+        auto live = [&](int u) { return FC ? ((wsum[u] != 0.0) | (w1sum[u] != 0.0)) : wsum[u] != 0.0; };
 #pragma unroll
         for (int u = 0; u < kP2; ++u) {
             const int j = base + u * kFpThreads + lane;
             kk[u] = 0;
-            if (wsum[u] != 0.0) {
+            if (live(u)) {
                 W[j] = (WCell)0;
+                if constexpr (FC) W1[j] = (WCell)0;
                 const int r = rr[u];
                 const int k = j - R.off(r);  // window slot
                 int pos = R.i0(r) + k;
@@ -2001,11 +2101,12 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
                 if (pos >= n) pos -= n;
                 kk[u] = k;
                 mv[u] = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
+                if constexpr (FC) mv1[u] = load_px<CS>(a.map1, (int64_t)R.first(r) + pos, a.npix);
             }
         }
 #pragma unroll
         for (int u = 0; u < kP2; ++u) {
-            if (wsum[u] == 0.0) continue;
+            if (!live(u)) continue;
             const int j = base + u * kFpThreads + lane;
             const int r = rr[u], k = kk[u];
             const double wsum_u = wsum[u];
@@ -2047,16 +2148,35 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
                     const Spin2Ring g = spin2_ring(R.z(r), R.sth(r), sp.z_pts, sp.sth_pts);
                     spin2_from_half(g, sh, ch, c2d, s2d);
                 }
-                const double qv = (double)mv_u.x, uv = (double)mv_u.y;
-                acc.v[0] = fma(wsum_u, fma(qv, c2d, uv * s2d), acc.v[0]);
-                acc.v[1] = fma(wsum_u, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
+                if constexpr (FC) {
+                    // both channels weighted first, then the spin-2 rotation
+                    const double w1 = w1sum[u];
+                    const float4 m1 = mv1[u];
+                    const double qv = fma(w1, (double)m1.x, wsum_u * (double)mv_u.x);
+                    const double uv = fma(w1, (double)m1.y, wsum_u * (double)mv_u.y);
+                    acc.v[0] += fma(qv, c2d, uv * s2d);
+                    acc.v[1] += fma(uv, c2d, -(qv * s2d));
 #pragma unroll
-                for (int k2 = 2; k2 < CS; ++k2)
-                    if (k2 < a.nslots) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
+                    for (int k2 = 2; k2 < CS; ++k2)
+                        if (k2 < a.nslots)
+                            acc.v[k2] += fma(w1, (double)slot(m1, k2), wsum_u * (double)slot(mv_u, k2));
+                } else {
+                    const double qv = (double)mv_u.x, uv = (double)mv_u.y;
+                    acc.v[0] = fma(wsum_u, fma(qv, c2d, uv * s2d), acc.v[0]);
+                    acc.v[1] = fma(wsum_u, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
+#pragma unroll
+                    for (int k2 = 2; k2 < CS; ++k2)
+                        if (k2 < a.nslots) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
+                }
             } else {
 #pragma unroll
-                for (int k2 = 0; k2 < CS; ++k2)
-                    if (k2 < a.nslots) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
+                for (int k2 = 0; k2 < CS; ++k2) {
+                    if (k2 >= a.nslots) continue;
+                    if constexpr (FC)
+                        acc.v[k2] += fma(w1sum[u], (double)slot(mv1[u], k2), wsum_u * (double)slot(mv_u, k2));
+                    else
+                        acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
+                }
             }
         }
     }
@@ -2098,7 +2218,7 @@ __device__ __forceinline__ void fp_flush(const TodArgs& a, const FpJob* J) {
     }
 }
 
-// Zero the weight grid and the side list.
+// Zero the weight grid (both channels' halves in mode 4) and the side list.
 __device__ __forceinline__ void fp_init() {
     for (int j = threadIdx.x; j < kFpCap; j += kFpThreads) fp_W()[j] = (WCell)0;
     if (threadIdx.x == 0) *fp_side_count() = 0;
@@ -2123,9 +2243,10 @@ __device__ __forceinline__ void fp_set_job(const TodArgs& a, int64_t b, int buf,
 
 // Wave 0: job and layout of sample b into buffer buf; a sample with a
 // non-finite operand gets an empty job (no beam pixels, no tod update).
-__device__ __forceinline__ void fp_prepare(const TodArgs& a, int64_t b, const BeamFrame& bf, int buf) {
+__device__ __forceinline__ void fp_prepare(const TodArgs& a, int64_t b, const BeamFrame& bf, int buf,
+                                           int cap = kFpCap) {
     fp_set_job(a, b, buf, bf.M);
-    const bool ok = fp_layout(a, b, bf, fp_frame(buf), fp_rings(buf), fp_rtab(buf));
+    const bool ok = fp_layout(a, b, bf, fp_frame(buf), fp_rings(buf), fp_rtab(buf), cap);
     if (!ok && threadIdx.x == 0) fp_job(buf)->S = 0;
 }
 
@@ -2144,7 +2265,7 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     const FpRings R = fp_rings(buf);
     const FpFrame* fr = fp_frame(buf);
     Acc4 acc = {{0.0, 0.0, 0.0, 0.0}};
-    if (wave == 0 && have_next) fp_prepare(an, bn, bfn, buf ^ 1);
+    if (wave == 0 && have_next) fp_prepare(an, bn, bfn, buf ^ 1, MODE == 4 ? kFpCapFc : kFpCap);
     // every wave, wave 0 once its layout is done: the job's claim counter
     // was zeroed when the job was set
     if constexpr (MODE == 3)
@@ -2152,10 +2273,10 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     else
         acc = fp_phase1<MODE, QU, CS, CLAIM, DUMP>(an, fp_job(buf), fr, R, buf, fp_W());
     __syncthreads();
-    acc = fp_phase2<QU, CS, P2>(an, fr, R, fp_rtab(buf), fp_W(), acc);
+    acc = fp_phase2<QU, CS, P2, MODE == 4>(an, fr, R, fp_rtab(buf), fp_W(), acc);
     // the pixels outside the layout, in pixel order (one thread)
     if (threadIdx.x == 0 && *fp_side_count() != 0)
-        acc = fp_side_flush<QU, CS>(fp_globals(an), fr, acc);
+        acc = fp_side_flush<QU, CS, MODE == 4>(fp_globals(an), fr, acc);
     // mode 3: the next sample's values go into the value grid, which no
     // wave reads between the barrier above and the one below
     if constexpr (MODE == 3)
@@ -2181,7 +2302,8 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     if (bbeg + kx >= bend) return;  // uniform over the block
     fp_init();
     // footprint margin in pixel sizes: bilinear and nearest neighbour rings
-    // lie within one ring spacing (<= 0.9) of the query, 4 x 4 corrector
+    // lie within one ring spacing (<= 0.9) of the query (the factorised
+    // corrector deposits into the bilinear neighbours too), 4 x 4 corrector
     // neighbourhoods within ~3.6, bicubic candidates within ~4
     const double margin = MODE == 2 ? 5.0 : (MODE == 3 ? 4.5 : 1.25);
     // the beam frame from the block's first sample with finite operands
@@ -2189,7 +2311,7 @@ __global__ __launch_bounds__(kFpThreads, MINW) void k_tod_fp(const TodArgs a) {
     while (bfs + per < bend && !finite_sample(a, bfs)) bfs += per;
     const BeamFrame bf = fp_beam_frame(beam_src(a), bfs, fp_scratch(), margin);
     int64_t b = bbeg + kx;
-    if ((threadIdx.x >> 6) == 0) fp_prepare(a, b, bf, 0);
+    if ((threadIdx.x >> 6) == 0) fp_prepare(a, b, bf, 0, MODE == 4 ? kFpCapFc : kFpCap);
     __syncthreads();
     CorrRegs C;
     unsigned long long n_corr = 0, n_fb = 0;
@@ -2391,7 +2513,7 @@ hipError_t launch_direct_cs(const TodArgs& a, int T, hipStream_t st) {
 // round, DUMP = the index-dump build.
 template <int MODE, bool QU, int CS, int MINW, int CLAIM = kClaim, int P2 = kP2Cells, bool DUMP = false>
 hipError_t launch_fp_cs(const TodArgs& a, hipStream_t st) {
-    const size_t smem = MODE == 3 ? kFpSmemBytesCorr : kFpSmemBytes;
+    const size_t smem = MODE == 3 ? kFpSmemBytesCorr : (MODE == 4 ? kFpSmemBytesFc : kFpSmemBytes);
     static std::once_flag once;
     std::call_once(once, [&] {
         (void)hipFuncSetAttribute((const void*)k_tod_fp<MODE, QU, CS, MINW, CLAIM, P2, DUMP>,
@@ -2473,6 +2595,9 @@ hipError_t launch_cs(const TodArgs& a, int variant, hipStream_t st) {
     if constexpr (MODE == 3) {
         // the corrector keeps its matrices in registers: the 2-waves build
         return launch_fp_cs<MODE, QU, CS, 2>(a, st);
+    } else if constexpr (MODE == 4) {
+        (void)variant;
+        return launch_fp_cs<MODE, QU, CS, 4>(a, st);
     } else {
         if constexpr (MODE < 2) {  // the bicubic judge runs on the footprint kernel only
             if (variant == 1 || variant == 2 || variant == 4)
@@ -2502,6 +2627,15 @@ hipError_t launch_m0(const TodArgs& a, bool qu, int CS, int variant, hipStream_t
 hipError_t launch_m1(const TodArgs& a, bool qu, int CS, int variant, hipStream_t st);
 hipError_t launch_m2(const TodArgs& a, bool qu, int CS, int variant, hipStream_t st);
 hipError_t launch_m3(const TodArgs& a, bool qu, int CS, hipStream_t st);
+hipError_t launch_m4(const TodArgs& a, bool qu, int CS, hipStream_t st);
+// The factorised corrector's stencil coefficients: c[0..15] build the main
+// channel, c[16..31] the correction channel (k_fcorr_channels, in
+// tod_fp_fcorr.hip).
+struct FcorrCoef {
+    double c[32];
+};
+hipError_t launch_fcorr_channels(const RingEntry* rings, int nside, const float* field, const FcorrCoef& c,
+                                 float* out0, float* out1, int64_t stride, hipStream_t st);
 hipError_t launch_batch_m0(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st);
 hipError_t launch_batch_m1(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st);
 hipError_t launch_batch_m2(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st);

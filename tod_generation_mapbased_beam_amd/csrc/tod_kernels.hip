@@ -368,6 +368,7 @@ int fill_args(const float* vec_orig, const float* axes, const float* cos_a, cons
     a->corr_count = 0;
     a->status = nullptr;
     a->dump = nullptr;
+    a->map1 = nullptr;
     *CS = slots_stride(nslots);
     return TODG_OK;
 }
@@ -420,6 +421,25 @@ int run_learned(const float* vec_orig, const float* axes, const float* cos_a, co
     a.corr_bound = bound;
     a.corr_count = n_corrected < S ? n_corrected : S;
     hipError_t e = launch_m3(a, qu, CS, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TODG_OK;
+}
+
+int run_fcorr(const float* vec_orig, const float* axes, const float* cos_a, const float* sin_a,
+              const float* ax_pts, const float* cos_p, const float* sin_p, int64_t nside,
+              const float* map_packed, const float* map1_packed, int C, const float* beam_vals,
+              int64_t B, int64_t S, double* tod, int c_q, int c_u, int* status, void* stream) {
+    TodArgs a;
+    bool qu;
+    int CS;
+    int rc = fill_args(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
+                       beam_vals, B, S, tod, c_q, c_u, true, &a, &qu, &CS);
+    if (rc) return rc;
+    if (B == 0 || S == 0) return TODG_OK;
+    if (!map1_packed) return fail(TODG_ERR_ARG, "null correction channel");
+    a.map1 = map1_packed;
+    a.status = status;
+    hipError_t e = launch_m4(a, qu, CS, reinterpret_cast<hipStream_t>(stream));
     if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     return TODG_OK;
 }
@@ -561,6 +581,35 @@ int todg_gather_accum_learned(const float* vec_orig, const float* axes, const fl
                               unsigned long long* stats, void* stream) {
     return run_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
                        beam_vals, B, S, tod, c_q, c_u, params, bound, n_corrected, stats, stream);
+}
+
+int todg_gather_accum_fcorr(const float* vec_orig, const float* axes, const float* cos_a,
+                            const float* sin_a, const float* ax_pts, const float* cos_p,
+                            const float* sin_p, int64_t nside, const float* map_packed,
+                            const float* map1_packed, int C, const float* beam_vals, int64_t B,
+                            int64_t S, double* tod, int c_q, int c_u, int* status, void* stream) {
+    return run_fcorr(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed,
+                     map1_packed, C, beam_vals, B, S, tod, c_q, c_u, status, stream);
+}
+
+int todg_fcorr_channels(int64_t nside, const float* field, const double* coef, float* out0,
+                        float* out1, int64_t stride, void* stream) {
+    if (!is_pow2(nside) || nside > 8192)
+        return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
+    if (!field || !coef || !out0 || !out1) return fail(TODG_ERR_ARG, "null pointer argument");
+    if (stride < 1) return fail(TODG_ERR_ARG, "stride must be >= 1");
+    FcorrCoef c;
+    for (int k = 0; k < 32; ++k) {
+        if (!isfinite(coef[k])) return fail(TODG_ERR_ARG, "non-finite stencil coefficient");
+        c.c[k] = coef[k];
+    }
+    const RingEntry* rings = nullptr;
+    int rc = get_ring_table(nside, &rings);
+    if (rc) return rc;
+    hipError_t e = launch_fcorr_channels(rings, (int)nside, field, c, out0, out1, stride,
+                                         reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TODG_OK;
 }
 
 int todg_interp_points(int mode, int64_t nside, const float* map, int64_t N, const double* z,
