@@ -191,9 +191,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a 1 GiB device-to-device copy (counter calibration)")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "day"], default="single",
                     help="single: one detector, one day (default); batch: 64 detectors with uneven "
-                         "beams and scan chunks in one persistent, graph-captured launch")
+                         "beams and scan chunks in one persistent, graph-captured launch; day: the "
+                         "drop-in driver (driver.tod_exact_gen_batched) end to end on a day of scan "
+                         "files: memmap reads, rotation operands, kernel, float32 rows summed into the "
+                         "float64 day, copy back to the host")
+    ap.add_argument("--prep", choices=["device", "host"], default="device",
+                    help="day: where the driver computes the rotation operands")
     ap.add_argument("--detectors", type=int, default=64)
     ap.add_argument("--batch-chunk", type=int, default=0,
                     help="batch: samples per claim; 0 (default) autotunes (build, chunk), otherwise "
@@ -216,6 +221,8 @@ def main():
     dev = torch.device("cuda", 0)
     if args.workload == "batch":
         return bench_batch(args, dev)
+    if args.workload == "day":
+        return bench_day(args, dev)
     vec, bv, params, mp = build_workload(args.nside, args.nside_beam, args.fwhm, args.samples,
                                          args.fs, 0, dev)
     B, S = args.samples, vec.shape[0]
@@ -384,6 +391,86 @@ def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fma
                         "rows": "tod rows 0, 1, 2 = I, Q, U", **j}
         log(f"learned: RMS residual vs bicubic / bilinear's = {j['rms_ratio']}, max {j['max_ratio']}")
     return out
+
+
+def bench_day(args, dev):
+    """One observation day through the drop-in driver: scan files of the
+    1.728M-sample ring scan written to a temporary folder, one beam entry
+    holding I, Q and U (Q/U beams equal to I), the whole day as one batch."""
+    import shutil
+    import tempfile
+
+    import torch
+
+    from tod_generation_mapbased_beam_amd import autotune, driver, inputs, kernels
+    from tod_generation_mapbased_beam_amd import tod_rotations as R
+    from tod_generation_mapbased_beam_amd.device import DeviceMap, to_dev
+
+    B = args.samples
+    vec, bv, _ = inputs.gaussian_beam(args.nside_beam, args.fwhm, 0.99)
+    S = vec.shape[0]
+    th, ph, ps = inputs.ring_scan(B, args.fs)
+    mp = inputs.white_map(args.nside, seed=0, fields=3, device="cpu").numpy()
+    folder = tempfile.mkdtemp(prefix="todg_day_")
+    try:
+        driver.write_scan_day(folder, 0, th, ph, ps)
+        bd = {"IQU": driver.beam_entry(vec, bv, [0, 1, 2], mp)}
+        # the autotuner's choice for this shape, as beam_tod_batch reads it
+        params = [to_dev(p, torch.float32) for p in R.rotation_params_for_pointing(th[:131072], ph[:131072],
+                                                                                    ps[:131072])]
+        tv, _ = autotune.tune_single(to_dev(vec, torch.float32), to_dev(bv, torch.float32), params,
+                                     DeviceMap(bd["IQU"]["mp_stacked"], 1, 2), mode="bilinear",
+                                     max_samples=131072, reps=3)
+        log(f"day: S={S} B={B}, tuned variant {tv}, prep {args.prep}")
+
+        def step():
+            return driver.tod_exact_gen_batched(bd, 0, mp, B, folder_scan=folder, log=None, prep=args.prep)
+
+        for i in range(args.warmup):
+            step()
+            log(f"warmup {i} done")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            tod = step()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # the device parts alone: rotation operands and the kernel
+        ctr = R.beam_centre(np.zeros((1, 1)), np.full((1, 1), np.pi / 2))
+        pts = [torch.as_tensor(x, device=dev) for x in (th, ph, ps)]
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        t = torch.zeros((3, B), dtype=torch.float64, device=dev)
+        e[0].record()
+        dp = kernels.rotation_params(*pts, ctr)
+        e[1].record()
+        kernels._gather_accum_fused_jit(to_dev(vec, torch.float32), *dp, args.nside,
+                                        kernels.device_map_for(bd["IQU"]["mp_stacked"], 1, 2),
+                                        to_dev(bv, torch.float32), B, S, t, 1, 2, variant=tv)
+        e[2].record()
+        torch.cuda.synchronize()
+    finally:
+        shutil.rmtree(folder, ignore_errors=True)
+    ms = wall * 1e3 / args.steps
+    work = float(B) * S
+    out = {
+        "metric": METRIC, "value": work / (ms / 1e3), "unit": UNIT, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": DTYPE, "data": "synthetic",
+        "config": {
+            "workload": f"one day through driver.tod_exact_gen_batched: {B} samples of a {args.fs:g} Hz ring "
+                        f"scan read from .npy memmaps, one I/Q/U beam entry ({args.fwhm:g}' Gaussian at "
+                        f"nside_beam {args.nside_beam}, 0.99 cut), nside {args.nside} map, whole day per batch, "
+                        f"rotation operands on the {args.prep}",
+            "nside": args.nside, "beam_pixels": S, "samples": B, "prep": args.prep, "kernel_variant": tv,
+            "output_finite": bool(np.isfinite(tod).all()),
+        },
+        "device_parts": {"rotation_params_ms": e[0].elapsed_time(e[1]), "kernel_ms": e[1].elapsed_time(e[2])},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    log(f"day: {ms:.1f} ms per day end to end; rotation operands {out['device_parts']['rotation_params_ms']:.2f} "
+        f"ms, kernel {out['device_parts']['kernel_ms']:.2f} ms")
+    print(json.dumps(out), flush=True)
 
 
 def bench_batch(args, dev):

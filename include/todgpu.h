@@ -283,6 +283,20 @@ int todg_gather_accum_learned(const float *vec_orig, const float *axes, const fl
                               int c_q, int c_u, const float *params, float bound,
                               int64_t n_corrected, unsigned long long *stats, void *stream);
 
+/* The six per-sample kernel operands of a batch of boresight pointings
+ * (device float64 theta, phi, psi, B each), computed on the GPU: the
+ * driver's float32 casts and psi_b - beta (sample_based_tod_generation_
+ * gridint.py:277-281), precompute_rotation_vector_batch (tod_rotations.py:
+ * 219-303) and _rotation_params (tod_rotations.py:306-333), each step in the
+ * precision the reference's numpy code takes it.  centre: HOST pointer to 6
+ * doubles, the beam centre's unit vector c and its local north n_c.  Outputs
+ * (device float32): axes (B, 3), cos_a, sin_a (B), ax_pts (B, 3), cos_p,
+ * sin_p (B).  Binary32 sines and cosines are the binary64 ones rounded once;
+ * numpy's float32 routines differ from these by at most one ulp. */
+int todg_rotation_params(const double *theta, const double *phi, const double *psi, int64_t B,
+                         const double *centre, float *axes, float *cos_a, float *sin_a,
+                         float *ax_pts, float *cos_p, float *sin_p, void *stream);
+
 /* Factorised corrector, map side: the two stencil channels of one map field
  * (field: npix float32, RING order).  Per pixel p on ring r, the 16 features
  * are F_0 = m(p) and, for (dr, ds) in {0..3}^2 minus (0, 0) in row-major

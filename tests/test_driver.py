@@ -39,8 +39,12 @@ def test_missing_day_is_reported_and_others_go_on(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prep", ["host", "device"])
 @pytest.mark.parametrize("batch_size", [10_000, 333])
-def test_days_match_oracle(tmp_path, batch_size):
+def test_days_match_oracle(tmp_path, batch_size, prep):
+    """Saved days against the oracle on the operands the driver used: the
+    numpy path's (prep="host"), or those todg_rotation_params computes on
+    the GPU (prep="device", downloaded)."""
     rng = np.random.default_rng(5)
     nside = 32
     mp = rng.standard_normal((3, 12 * nside * nside)).astype(np.float32)
@@ -53,17 +57,24 @@ def test_days_match_oracle(tmp_path, batch_size):
         th, ph, ps = inputs.ring_scan(n, 10.0, t0=1000.0 * d)
         driver.write_scan_day(str(tmp_path / "scan"), d, th, ph, ps)
     res = driver.run_days(bd, mp, days + [7], batch_size, str(tmp_path / "scan"), str(tmp_path / "out"),
-                          log=None)
+                          log=None, prep=prep)
     assert [ok for _, ok, _ in res] == [True, True, False]  # day 7 has no scan files
     S = vec.shape[0]
     for d in days:
         got = np.load(tmp_path / "out" / f"tod_day_{d}.npy")
         assert got.shape == (3, n) and got.dtype == np.float64
         th, ph, ps = driver.open_scan_day(str(tmp_path / "scan"), d)
-        phi_b, theta_b, psi_b = (np.array(x, dtype=np.float32) for x in (ph, th, ps))
-        rv, betas = R.precompute_rotation_vector_batch(np.zeros((1, 1)), np.full((1, 1), math.pi / 2),
-                                                      phi_b, theta_b)
-        params = R._rotation_params(rv, phi_b, theta_b, -betas + psi_b)
+        if prep == "host":
+            phi_b, theta_b, psi_b = (np.array(x, dtype=np.float32) for x in (ph, th, ps))
+            rv, betas = R.precompute_rotation_vector_batch(np.zeros((1, 1)), np.full((1, 1), math.pi / 2),
+                                                          phi_b, theta_b)
+            params = R._rotation_params(rv, phi_b, theta_b, -betas + psi_b)
+        else:
+            from tod_generation_mapbased_beam_amd import kernels
+
+            ctr = R.beam_centre(np.zeros((1, 1)), np.full((1, 1), math.pi / 2))
+            params = [p.cpu().numpy() for p in kernels.rotation_params(np.array(th), np.array(ph),
+                                                                      np.array(ps), ctr)]
         ref = np.zeros((3, n))
         t1 = np.zeros((1, n))
         O.gather_accum_fused(vec, *params, nside, mp[[0]], bv, n, S, t1, -1, -1)

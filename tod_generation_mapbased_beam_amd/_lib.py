@@ -64,6 +64,7 @@ SIGNATURES = {
     ),
     "todg_corr_gemm_bf16": (_i32, [_i32, _i32, _i32, _p, _i32, _i32, _p, _i32, _i32, _p, _i32,
                                     ctypes.c_float, ctypes.c_float, _p]),
+    "todg_rotation_params": (_i32, [_p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p]),
     "todg_fcorr_channels": (_i32, [_i64, _p, _p, _p, _p, _i64, _p]),
     "todg_gather_accum_fcorr": (
         _i32,

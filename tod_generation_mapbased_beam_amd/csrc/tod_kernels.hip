@@ -278,6 +278,85 @@ __global__ __launch_bounds__(256) void k_pack_map(const float* __restrict__ src,
         dst[CS * i + k] = perm[k] >= 0 ? src[(int64_t)perm[k] * npix + i] : 0.f;
 }
 
+// Beam centre c (unit vector) and its local north n_c, binary64.
+struct RotCentre {
+    double c[3];
+    double n[3];
+};
+
+// Binary32 sin / cos as the binary64 function rounded once (the host's
+// float32 routines differ from this by at most one ulp).
+__device__ __forceinline__ float sin_f32(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cos_f32(float x) { return (float)cos((double)x); }
+
+// Per-sample rotation operands from boresight pointings, one thread per
+// sample: the driver's casts (sample_based_tod_generation_gridint.py:277-281:
+// pointings to binary32, psi_b - beta), the Rodrigues vector and parallactic
+// offset of precompute_rotation_vector_batch (tod_rotations.py:219-303) and
+// the kernel operands of _rotation_params (tod_rotations.py:306-333), with
+// each step in the precision those take: binary32 unit vectors of the
+// binary32 angles, binary64 cross products, arccos, rotation and arctan2,
+// then the binary32 angle, axis, boresight and roll.  ctr = beam centre c
+// and its local north n_c (binary64, from the host).
+__global__ __launch_bounds__(256) void k_rotation_params(const double* __restrict__ theta,
+                                                         const double* __restrict__ phi,
+                                                         const double* __restrict__ psi, int64_t B,
+                                                         RotCentre ctr, float* __restrict__ axes,
+                                                         float* __restrict__ cos_a, float* __restrict__ sin_a,
+                                                         float* __restrict__ ax_pts, float* __restrict__ cos_p,
+                                                         float* __restrict__ sin_p) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const float th = (float)theta[b], ph = (float)phi[b], ps = (float)psi[b];
+    const float st = sin_f32(th), ct = cos_f32(th), sp = sin_f32(ph), cp = cos_f32(ph);
+    // boresight t, its north n_t and east e_t (binary32)
+    const float tx = st * cp, ty = st * sp, tz = ct;
+    const float nx = cp * ct, ny = sp * ct, nz = -st;
+    const float ex = -sp, ey = cp;
+    const double cx = ctr.c[0], cy = ctr.c[1], cz = ctr.c[2];
+    double kx = cy * (double)tz - cz * (double)ty;
+    double ky = cz * (double)tx - cx * (double)tz;
+    double kz = cx * (double)ty - cy * (double)tx;
+    const double kn = sqrt(kx * kx + ky * ky + kz * kz);
+    if (kn > 1e-10) {
+        kx /= kn;
+        ky /= kn;
+        kz /= kn;
+    } else {
+        kx = ky = kz = 0.0;
+    }
+    const double dot = cx * (double)tx + cy * (double)ty + cz * (double)tz;
+    const double ang = acos(fmax(-1.0, fmin(1.0, dot)));
+    const double rx = kx * ang, ry = ky * ang, rz = kz * ang;
+    // the centre's north rotated by the same Rodrigues rotation, and its
+    // bearing in the boresight's (north, east) frame
+    const double ca = cos(ang), sa = sin(ang);
+    const double vx = ctr.n[0], vy = ctr.n[1], vz = ctr.n[2];
+    const double kv = kx * vx + ky * vy + kz * vz;
+    const double wx = vx * ca + (ky * vz - kz * vy) * sa + kx * kv * (1.0 - ca);
+    const double wy = vy * ca + (kz * vx - kx * vz) * sa + ky * kv * (1.0 - ca);
+    const double wz = vz * ca + (kx * vy - ky * vx) * sa + kz * kv * (1.0 - ca);
+    double beta = atan2(wx * (double)ex + wy * (double)ey + wz * 0.0,
+                        wx * (double)nx + wy * (double)ny + wz * (double)nz);
+    if (beta < 0.0) beta += 2.0 * M_PI;
+    const double psis = -beta + (double)ps;
+    // kernel operands
+    const float a32 = (float)sqrt(rx * rx + ry * ry + rz * rz);
+    const bool ok = a32 > 1e-10f;
+    const double den = ok ? (double)a32 : 1.0;
+    axes[3 * b] = ok ? (float)(rx / den) : 0.f;
+    axes[3 * b + 1] = ok ? (float)(ry / den) : 0.f;
+    axes[3 * b + 2] = ok ? (float)(rz / den) : 0.f;
+    cos_a[b] = cos_f32(a32);
+    sin_a[b] = sin_f32(a32);
+    ax_pts[3 * b] = st * cp;
+    ax_pts[3 * b + 1] = st * sp;
+    ax_pts[3 * b + 2] = ct;
+    const float p32 = (float)psis;
+    cos_p[b] = cos_f32(p32);
+    sin_p[b] = sin_f32(p32);
+}
+
 // ============================================================ host helpers
 
 int slots_stride(int nslots) { return nslots <= 1 ? 1 : (nslots == 2 ? 2 : 4); }
@@ -581,6 +660,27 @@ int todg_gather_accum_learned(const float* vec_orig, const float* axes, const fl
                               unsigned long long* stats, void* stream) {
     return run_learned(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, map_packed, C,
                        beam_vals, B, S, tod, c_q, c_u, params, bound, n_corrected, stats, stream);
+}
+
+int todg_rotation_params(const double* theta, const double* phi, const double* psi, int64_t B,
+                         const double* centre, float* axes, float* cos_a, float* sin_a, float* ax_pts,
+                         float* cos_p, float* sin_p, void* stream) {
+    if (B < 0) return fail(TODG_ERR_ARG, "B must be non-negative");
+    if (B == 0) return TODG_OK;
+    if (!theta || !phi || !psi || !centre || !axes || !cos_a || !sin_a || !ax_pts || !cos_p || !sin_p)
+        return fail(TODG_ERR_ARG, "null pointer argument");
+    RotCentre ctr;
+    for (int k = 0; k < 3; ++k) {
+        ctr.c[k] = centre[k];
+        ctr.n[k] = centre[3 + k];
+    }
+    for (int k = 0; k < 3; ++k)
+        if (!isfinite(ctr.c[k]) || !isfinite(ctr.n[k])) return fail(TODG_ERR_ARG, "non-finite beam centre");
+    hipLaunchKernelGGL(k_rotation_params, dim3((unsigned)((B + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), theta, phi, psi, B, ctr, axes, cos_a, sin_a,
+                       ax_pts, cos_p, sin_p);
+    TODG_HIP_CHECK(hipGetLastError());
+    return TODG_OK;
 }
 
 int todg_gather_accum_fcorr(const float* vec_orig, const float* axes, const float* cos_a,

@@ -8,9 +8,12 @@ casts and calls (pointings to float32, ``precompute_rotation_vector_batch``,
 results summed into a float64 day array).  The map stays packed in HBM for
 the whole run (it replaces the reference's POSIX shared-memory copy,
 sample_based_tod_generation_gridint.py:478-507), so a batch can be a whole
-day; the host prepares the next batch's rotation operands while the GPU
-works on the current one.  A failing day is reported and the others go on
-(sample_based_tod_generation_gridint.py:319-333).
+day.  The rotation operands are computed on the GPU from the uploaded
+pointings (``prep="device"``, the default: ``todg_rotation_params``, within
+one float32 ulp of the numpy path) and the day's TOD is summed in HBM; with
+``prep="host"`` they come from the numpy path, the host preparing the next
+batch while the GPU works on the current one.  A failing day is reported and
+the others go on (sample_based_tod_generation_gridint.py:319-333).
 """
 
 from __future__ import annotations
@@ -22,7 +25,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from .tod_core import beam_tod_batch
-from .tod_rotations import precompute_rotation_vector_batch
+from .tod_rotations import beam_centre, precompute_rotation_vector_batch
 
 
 def open_scan_day(folder_scan: str, day_index: int):
@@ -65,10 +68,36 @@ def _host_batch(theta_mmap, phi_mmap, psi_mmap, bs, be, ra0, dec0):
     return rot_vecs, phi_b, theta_b, psis_b
 
 
+def _device_day(beam_data, nside, mp, theta_mmap, phi_mmap, psi_mmap, n_samples, batch_size, ra0, dec0,
+                interp_mode):
+    import torch
+
+    from . import kernels
+    from .device import device
+
+    dev = device()
+    ctr = beam_centre(ra0, dec0)
+    tod_day = torch.zeros((3, n_samples), dtype=torch.float64, device=dev)
+    for bs in range(0, n_samples, batch_size):
+        be = min(bs + batch_size, n_samples)
+        pts = [torch.from_numpy(np.array(m[bs:be], dtype=np.float64)).to(dev)  # (a writeable copy)
+               for m in (theta_mmap, phi_mmap, psi_mmap)]
+        params = kernels.rotation_params(*pts, ctr)
+        for data in beam_data.values():
+            contrib = beam_tod_batch(nside, mp, data, None, None, None, None, interp_mode=interp_mode,
+                                     params=params)
+            for comp, vals in contrib.items():
+                tod_day[comp, bs:be] += vals
+    return tod_day.cpu().numpy()
+
+
 def tod_exact_gen_batched(beam_data, day_index, mp, batch_size, process_name=None, folder_scan=".",
-                          interp_mode="bilinear", log=print):
+                          interp_mode="bilinear", log=print, prep="device"):
     """TOD of one observation day, shape (3, n_samples) float64 (axis 0 =
-    I, Q, U), in ceil(n_samples / batch_size) batches."""
+    I, Q, U), in ceil(n_samples / batch_size) batches; ``prep`` = where the
+    rotation operands are computed ("device" or "host")."""
+    if prep not in ("device", "host"):
+        raise ValueError("prep must be 'device' or 'host'")
     prefix = f"[{process_name}] " if process_name else ""
     npix = int(np.asarray(mp[0]).shape[0])
     nside = int(round((npix / 12) ** 0.5))
@@ -80,8 +109,15 @@ def tod_exact_gen_batched(beam_data, day_index, mp, batch_size, process_name=Non
     n_batches = (n_samples + batch_size - 1) // batch_size
     if log:
         log(prefix + f"Day {day_index} - {n_samples} samples, batch_size={batch_size}, n_batches={n_batches}")
-    tod_day = np.zeros((3, n_samples))
     t0 = time.time()
+    if prep == "device":
+        tod_day = _device_day(beam_data, nside, mp, theta_mmap, phi_mmap, psi_mmap, n_samples, batch_size,
+                              ra0, dec0, interp_mode)
+        if log:
+            total = time.time() - t0
+            log(prefix + f"Done - {n_samples} samples in {total:.2f}s ({total / n_batches:.3f}s/batch)")
+        return tod_day
+    tod_day = np.zeros((3, n_samples))
     with ThreadPoolExecutor(max_workers=1) as pool:
         nxt = pool.submit(_host_batch, theta_mmap, phi_mmap, psi_mmap, 0, min(batch_size, n_samples),
                           ra0, dec0)
@@ -104,7 +140,7 @@ def tod_exact_gen_batched(beam_data, day_index, mp, batch_size, process_name=Non
 
 
 def run_days(beam_data, mp, days, batch_size, folder_scan, folder_out, interp_mode="bilinear",
-             log=print):
+             log=print, prep="device"):
     """Generate and save tod_day_<i>.npy for every day; a day that fails is
     reported as (day, False, message) and the others go on.  Returns the list
     of (day, ok, error) tuples."""
@@ -113,7 +149,7 @@ def run_days(beam_data, mp, days, batch_size, folder_scan, folder_out, interp_mo
     for d in days:
         try:
             tod = tod_exact_gen_batched(beam_data, d, mp, batch_size, folder_scan=folder_scan,
-                                        interp_mode=interp_mode, log=log)
+                                        interp_mode=interp_mode, log=log, prep=prep)
             np.save(os.path.join(folder_out, f"tod_day_{d}.npy"), tod)
             results.append((d, True, None))
         except Exception as e:  # failure isolation per day, as the reference's worker

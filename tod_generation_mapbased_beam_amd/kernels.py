@@ -224,3 +224,27 @@ def nearest_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, B
         )
     )
     return pix.cpu().numpy()
+
+
+def rotation_params(theta, phi, psi, centre):
+    """The six float32 kernel operands of boresight pointings (float64, numpy
+    or torch, B each) computed on the GPU (``todg_rotation_params``: the
+    driver's casts, precompute_rotation_vector_batch and _rotation_params,
+    tod_rotations.py:219-333).  ``centre``: tod_rotations.beam_centre(...)."""
+    f64 = torch.float64
+    th, ph, ps = (to_dev(x, f64).reshape(-1) for x in (theta, phi, psi))
+    B = int(th.numel())
+    if ph.numel() != B or ps.numel() != B:
+        raise ValueError("theta, phi and psi must have the same length")
+    ctr = np.ascontiguousarray(centre, dtype=np.float64).reshape(-1)
+    if ctr.shape != (6,):
+        raise ValueError("centre must hold 6 values (c, n_c)")
+    f32 = torch.float32
+    dev = th.device
+    axes = torch.empty((B, 3), dtype=f32, device=dev)
+    ax_pts = torch.empty((B, 3), dtype=f32, device=dev)
+    cos_a, sin_a, cos_p, sin_p = (torch.empty(B, dtype=f32, device=dev) for _ in range(4))
+    _lib.check(_lib.lib().todg_rotation_params(
+        _lib.ptr(th), _lib.ptr(ph), _lib.ptr(ps), B, ctr.ctypes.data, _lib.ptr(axes), _lib.ptr(cos_a),
+        _lib.ptr(sin_a), _lib.ptr(ax_pts), _lib.ptr(cos_p), _lib.ptr(sin_p), stream_handle()))
+    return axes, cos_a, sin_a, ax_pts, cos_p, sin_p

@@ -29,7 +29,7 @@ def _qu_slots(comp_indices):
 
 
 def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mode="bilinear",
-                   variant=None):
+                   variant=None, params=None):
     """Beam-weighted sky samples of one beam entry for a batch of pointings.
 
     Args mirror tod_core.py:52-96: ``data`` needs ``vec_orig (S,3)``,
@@ -37,7 +37,10 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
     (C, npix)`` float32 (numpy, torch, or a packed ``DeviceMap``).
     ``interp_mode`` is ``"bilinear"`` or ``"nearest"`` (tod_core.py:114-119),
     or ``"bicubic"`` (the reference's bicubic branch, README.md:120, 317);
-    anything else raises ``ValueError`` as the reference does.
+    anything else raises ``ValueError`` as the reference does.  ``params``:
+    the six kernel operands already computed for this batch (e.g. on the GPU
+    by ``kernels.rotation_params``); rot_vecs / phi_b / theta_b / psis_b are
+    then not used (pass None) and the result rows stay on the device.
 
     When ``mp_stacked`` is absent the reference falls back to a split
     healpy pipeline with bilinear weights and no spin-2 (tod_core.py:169-180);
@@ -48,7 +51,7 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
     vec_orig = data["vec_orig"]
     beam_vals = data["beam_vals"]
     comp_indices = list(data["comp_indices"])
-    B = int(phi_b.shape[0])
+    B = int(phi_b.shape[0]) if params is None else int(params[1].shape[0])
     S = int(vec_orig.shape[0])
     C = len(comp_indices)
     mp_stacked = data.get("mp_stacked")
@@ -59,11 +62,12 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
             f"interp_mode {interp_mode!r} not available; use 'bilinear', 'nearest' or 'bicubic'"
         )
 
-    on_device = isinstance(rot_vecs, torch.Tensor)
-    if on_device:
-        params = rotation_params_torch(rot_vecs, phi_b, theta_b, psis_b)
-    else:
-        params = _rotation_params(rot_vecs, phi_b, theta_b, psis_b)
+    on_device = params is not None or isinstance(rot_vecs, torch.Tensor)
+    if params is None:
+        if on_device:
+            params = rotation_params_torch(rot_vecs, phi_b, theta_b, psis_b)
+        else:
+            params = _rotation_params(rot_vecs, phi_b, theta_b, psis_b)
 
     if mp_stacked is None:
         mp_stacked = torch.stack([to_dev(mp[c], torch.float32) for c in comp_indices])

@@ -18,6 +18,24 @@ def _unit(phi, theta):
     return np.stack([st * np.cos(phi), st * np.sin(phi), np.cos(theta)], axis=-1)
 
 
+def beam_centre(ra, dec, center_idx=None) -> np.ndarray:
+    """(6,) float64: the beam centre's unit vector c and its local north n_c
+    for beam-frame offset grids ``ra``/``dec`` (centre element ``center_idx``,
+    default ``(H//2, W//2)``), as precompute_rotation_vector_batch builds
+    them (tod_rotations.py:219-303)."""
+    ra = np.asarray(ra, dtype=np.float64)
+    dec = np.asarray(dec, dtype=np.float64)
+    if center_idx is None:
+        center_idx = (ra.shape[0] // 2, ra.shape[1] // 2)
+    phi_c = ra[center_idx]
+    theta_c = np.pi / 2 - dec[center_idx]
+    c = _unit(phi_c, theta_c)
+    n_c = np.array(
+        [np.cos(phi_c) * np.cos(theta_c), np.sin(phi_c) * np.cos(theta_c), -np.sin(theta_c)]
+    )
+    return np.concatenate([c, n_c])
+
+
 def precompute_rotation_vector_batch(ra, dec, phi_batch, theta_batch, center_idx=None):
     """Rodrigues vectors taking the beam centre to each boresight, and the
     parallactic offsets beta in [0, 2pi).  Same contract as
@@ -27,22 +45,14 @@ def precompute_rotation_vector_batch(ra, dec, phi_batch, theta_batch, center_idx
 
     A HEALPix beam centred on the north pole is ``ra=[[0.]], dec=[[pi/2]]``.
     """
-    ra = np.asarray(ra, dtype=np.float64)
-    dec = np.asarray(dec, dtype=np.float64)
-    if center_idx is None:
-        center_idx = (ra.shape[0] // 2, ra.shape[1] // 2)
-    phi_c = ra[center_idx]
-    theta_c = np.pi / 2 - dec[center_idx]
-
+    ctr = beam_centre(ra, dec, center_idx)
     phi_b = np.asarray(phi_batch)
     theta_b = np.asarray(theta_batch)
-    c = _unit(phi_c, theta_c)[None, :]  # (1, 3) beam centre
+    c = ctr[None, :3]  # (1, 3) beam centre
     t = _unit(phi_b, theta_b)  # (B, 3) boresight
 
     # Local north at the beam centre and local north/east at the boresight.
-    n_c = np.array(
-        [np.cos(phi_c) * np.cos(theta_c), np.sin(phi_c) * np.cos(theta_c), -np.sin(theta_c)]
-    )
+    n_c = ctr[3:]
     n_t = np.stack(
         [np.cos(phi_b) * np.cos(theta_b), np.sin(phi_b) * np.cos(theta_b), -np.sin(theta_b)],
         axis=-1,
