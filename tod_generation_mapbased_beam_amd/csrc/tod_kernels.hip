@@ -1,31 +1,18 @@
-// Beam-convolution TOD kernels for gfx950 and their C-ABI entry points.
+// Host side of the beam-convolution TOD library for gfx950: the C-ABI
+// entry points of include/todgpu.h, argument checks and launch dispatch, the
+// per-device ring and reference-longitude tables, and the small kernels that
+// are not per-mode (index-path dumps, point interpolation, map packing, the
+// rotation operands).
 //
 //   tod[c, b] += sum_s beam_vals[s] * spin2_c( map[ R_b * vec_orig[s] ] )
 //
-// Two kernels share one device index path (binary64, reference operation
-// order, see healpix_ring.hpp):
-//
-// * k_tod_fp (default): one workgroup per sample at a time.  The sample's
-//   footprint on the map -- the rings and ring slots that the rotated beam
-//   and its bilinear neighbours can touch -- is laid out in LDS.  Phase 1
-//   sweeps the beam: for every beam pixel it evaluates the rotated direction
-//   and its neighbours and adds beam_val * weight into a per-pixel LDS
-//   weight grid (ds_add_f64).  Phase 2 sweeps the footprint once: per map
-//   pixel it computes the spin-2 factors and gathers I/Q/U with one 16-byte
-//   load.  Spin-2 and the map gather therefore run once per footprint pixel
-//   (~2.3k per sample at nside 2048 / 30') instead of once per neighbour
-//   (4 x S ~ 25k), which is the per-sample spin-2 memo of the reference
-//   (tod_bilinear.py:151-199) taken to its limit.  A neighbour that falls
-//   outside the laid-out footprint is accumulated directly (spin-2 + gather
-//   on the spot), so the result never depends on the footprint estimate.
-//
-// * k_tod_direct: one wavefront owns T consecutive samples and sweeps the
-//   beam with its lanes, evaluating spin-2 and gathering per neighbour.
-//   Kept as a cross-check and for callers that pick it explicitly.
-//
-// Both keep binary64 sums per lane (binary64 adds issue at the full VALU
-// rate on CDNA4) and reduce across lanes once per sample; each sample's tod
-// entry is written by exactly one thread, so there are no atomics on tod.
+// The TOD kernels themselves -- the footprint kernel k_tod_fp (per-sample
+// footprint and weight grid in LDS, certified fast index path, one gather
+// per footprint pixel), its batched persistent form k_tod_fp_batch and the
+// direct kernel k_tod_direct -- are in tod_device.hpp; each mode's template
+// instantiations are compiled in their own unit (tod_fp_bilinear.hip,
+// tod_fp_nearest.hip, tod_fp_bicubic.hip, tod_fp_learned.hip,
+// tod_fp_fcorr.hip) and reached through the launch_m* functions.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
