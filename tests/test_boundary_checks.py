@@ -94,3 +94,46 @@ def test_batch_plan_rejects_host_tod(args):
         det = Detector(vec, bv, params, tod=BAD_TODS[bad](3, B))
         with pytest.raises(ValueError):
             BatchPlan([det], FakeMap())
+
+
+class _FakeMaps:
+    """Stands in for an FcorrMaps: the checks must fail before its buffers
+    are touched."""
+    C = 3
+    nside = 4
+    npix = 192
+    c_q, c_u = 1, 2
+    packed0 = packed1 = None
+
+
+@pytest.mark.parametrize("bad", sorted(BAD_TODS))
+def test_fcorr_entry_rejects_bad_tod(args, bad):
+    from tod_generation_mapbased_beam_amd import fcorr
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        fcorr.gather_accum_fcorr(vec, *params, nside, _FakeMaps(), bv, B, S, BAD_TODS[bad](3, B))
+
+
+def test_fcorr_entry_rejects_a_host_array(args):
+    from tod_generation_mapbased_beam_amd import fcorr
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        fcorr.gather_accum_fcorr(vec, *params, nside, _FakeMaps(), bv, B, S, np.zeros((3, B)))
+
+
+@pytest.mark.parametrize("case", ["lengths", "centre size", "centre nan"])
+def test_rotation_params_rejects_bad_arguments(case):
+    from tod_generation_mapbased_beam_amd import kernels
+
+    th = np.zeros(5)
+    ctr = np.array([0.0, 0.0, 1.0, 1.0, 0.0, 0.0])
+    if case == "lengths":
+        a = (th, np.zeros(4), np.zeros(5), ctr)
+    elif case == "centre size":
+        a = (th, th, th, ctr[:5])
+    else:
+        a = (th, th, th, np.where(np.arange(6) == 2, np.nan, ctr))
+    with pytest.raises(ValueError):
+        kernels.rotation_params(*a)

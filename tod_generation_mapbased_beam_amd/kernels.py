@@ -231,14 +231,15 @@ def rotation_params(theta, phi, psi, centre):
     or torch, B each) computed on the GPU (``todg_rotation_params``: the
     driver's casts, precompute_rotation_vector_batch and _rotation_params,
     tod_rotations.py:219-333).  ``centre``: tod_rotations.beam_centre(...)."""
-    f64 = torch.float64
-    th, ph, ps = (to_dev(x, f64).reshape(-1) for x in (theta, phi, psi))
-    B = int(th.numel())
-    if ph.numel() != B or ps.numel() != B:
+    sizes = [int(x.numel()) if isinstance(x, torch.Tensor) else int(np.asarray(x).size) for x in (theta, phi, psi)]
+    if len(set(sizes)) != 1:
         raise ValueError("theta, phi and psi must have the same length")
     ctr = np.ascontiguousarray(centre, dtype=np.float64).reshape(-1)
-    if ctr.shape != (6,):
-        raise ValueError("centre must hold 6 values (c, n_c)")
+    if ctr.shape != (6,) or not np.all(np.isfinite(ctr)):
+        raise ValueError("centre must hold 6 finite values (c, n_c)")
+    f64 = torch.float64
+    th, ph, ps = (to_dev(x, f64).reshape(-1) for x in (theta, phi, psi))
+    B = sizes[0]
     f32 = torch.float32
     dev = th.device
     axes = torch.empty((B, 3), dtype=f32, device=dev)
