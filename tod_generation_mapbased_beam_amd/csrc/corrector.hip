@@ -67,6 +67,66 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const fl
     }
 }
 
+// Split-K form of k_gemm_bf16 for products with few output tiles and a long
+// K (the weight gradients, K = batch rows): wave w takes tile w % tiles and
+// the K range [s kc, (s + 1) kc) of split s = w / tiles, and writes its
+// 16 x 16 partial tile (MFMA layout: lane l, element q) to
+// P[(s tiles + tile) 256 + 4 l + q]; k_splitk_reduce adds the splits in
+// order, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_gemm_bf16_splitk(int M, int N, int K, int kc, int splits,
+                                                          const float* __restrict__ A, int lda, int ta,
+                                                          const float* __restrict__ B, int ldb, int tb,
+                                                          float* __restrict__ P) {
+    const int lane = threadIdx.x & 63;
+    const int tiles_n = (N + 15) / 16;
+    const int tiles = ((M + 15) / 16) * tiles_n;
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= (int64_t)tiles * splits) return;
+    const int tile = (int)(w % tiles), sp = (int)(w / tiles);
+    const int ti = tile / tiles_n, tj = tile % tiles_n;
+    const int ia = ti * 16 + (lane & 15);
+    const int jb = tj * 16 + (lane & 15);
+    const int kq = (lane >> 4) * 4;
+    const int kbeg = sp * kc;
+    const int kend = kbeg + kc < K ? kbeg + kc : K;
+    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
+        short4_t av, bv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + kq + q;
+            float x = 0.f, y = 0.f;
+            if (ia < M && k < kend) x = ta ? A[(int64_t)k * lda + ia] : A[(int64_t)ia * lda + k];
+            if (jb < N && k < kend) y = tb ? B[(int64_t)jb * ldb + k] : B[(int64_t)k * ldb + jb];
+            av[q] = bf16_bits(x);
+            bv[q] = bf16_bits(y);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    float* o = P + ((int64_t)sp * tiles + tile) * 256 + 4 * lane;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = acc[q];
+}
+
+// C[i][j] = alpha * sum_s P[s][tile(i, j)][lane(i, j), q(i, j)] + beta * C[i][j],
+// splits added in order.
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int splits,
+                                                       const float* __restrict__ P, float* __restrict__ C,
+                                                       int ldc, float alpha, float beta) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * N) return;
+    const int i = (int)(e / N), j = (int)(e % N);
+    const int tiles_n = (N + 15) / 16;
+    const int tiles = ((M + 15) / 16) * tiles_n;
+    const int tile = (i / 16) * tiles_n + j / 16;
+    const int r = i % 16, c = j % 16;
+    const int off = 4 * ((r / 4) * 16 + c) + r % 4;  // C[4(l/16) + q][l%16]
+    float sum = 0.f;
+    for (int sp = 0; sp < splits; ++sp) sum += P[((int64_t)sp * tiles + tile) * 256 + off];
+    float* out = C + (int64_t)i * ldc + j;
+    *out = beta == 0.f ? alpha * sum : fmaf(alpha, sum, beta * *out);
+}
+
 // Z[i][f] += b[f]; A = gelu(Z) (Z kept for the backward pass).
 __global__ __launch_bounds__(256) void k_bias_gelu(int64_t rows, int F, const float* __restrict__ b,
                                                    float* __restrict__ Z, float* __restrict__ A) {
@@ -264,9 +324,27 @@ int fail(int code, const std::string& msg) { return todg_internal::set_error(cod
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+// K per split of the split-K products, and the largest tile count they take
+// (the 64 x 64 weight gradient).
+constexpr int kSplitKc = 256;
+constexpr int kSplitMaxTiles = (kH / 16) * (kH / 16);
+
+inline int64_t n_splits(int64_t K) { return (K + kSplitKc - 1) / kSplitKc; }
+
+// part: workspace for the split-K partials (n_splits(K) * tiles * 256
+// floats), or null for the one-wave-per-tile kernel.
 void gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, int ta, const float* B,
-          int ldb, int tb, float* C, int ldc, float alpha, float beta) {
+          int ldb, int tb, float* C, int ldc, float alpha, float beta, float* part = nullptr) {
     const int64_t tiles = (int64_t)((M + 15) / 16) * ((N + 15) / 16);
+    if (part && K >= 4 * kSplitKc && tiles <= kSplitMaxTiles) {
+        const int splits = (int)n_splits(K);
+        const int64_t waves = tiles * splits;
+        hipLaunchKernelGGL(k_gemm_bf16_splitk, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, M, N, K,
+                           kSplitKc, splits, A, lda, ta, B, ldb, tb, part);
+        hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)(((int64_t)M * N + 255) / 256)), dim3(256), 0, st,
+                           M, N, splits, part, C, ldc, alpha, beta);
+        return;
+    }
     hipLaunchKernelGGL(k_gemm_bf16, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, st, M, N, K,
                        A, lda, ta, B, ldb, tb, C, ldc, alpha, beta);
 }
@@ -289,7 +367,12 @@ using namespace todg_corr;
 
 extern "C" {
 
-size_t todg_corr_workspace_floats(int64_t N) { return (size_t)N * (4 * kH + 2 * kOut + 1) + 1024; }
+// Z1, A1, Z2, A2 (N x kH each), O, dO (N x kOut), the head's per-block loss
+// parts (doubles, within N floats), then the split-K partials of the weight
+// gradients.
+size_t todg_corr_workspace_floats(int64_t N) {
+    return (size_t)N * (4 * kH + 2 * kOut + 1) + 1024 + (size_t)n_splits(N) * kSplitMaxTiles * 256;
+}
 
 int todg_corr_features(int64_t nside, const float* field, int64_t N, const double* z,
                        const double* phi, float* X, float* wbil, float* mu, float* sigma, int* pix,
@@ -332,6 +415,7 @@ int todg_corr_loss_grad(const float* params, const float* X, const float* wbil, 
     float* O = A2 + N * kH;
     float* dO = O + N * kOut;
     double* part = reinterpret_cast<double*>(dO + N * kOut);  // 8-byte aligned: 288 N floats
+    float* spl = dO + N * kOut + N + 1024;                     // split-K partials
     forward(st, params, X, N, Z1, A1, Z2, A2, O);
     const unsigned hb = nblk(N);
     hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, N, X, wbil, that, O, grad ? dO : nullptr,
@@ -339,17 +423,17 @@ int todg_corr_loss_grad(const float* params, const float* X, const float* wbil, 
     if (loss) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (int)hb, part, loss);
     if (grad) {
         // layer 3: dW3 = dO^T A2, db3 = colsum dO, dA2 = dO W3
-        gemm(st, kOut, kH, n, dO, kOut, 1, A2, kH, 0, grad + kOffW3, kH, 1.f, 0.f);
+        gemm(st, kOut, kH, n, dO, kOut, 1, A2, kH, 0, grad + kOffW3, kH, 1.f, 0.f, spl);
         hipLaunchKernelGGL(k_colsum, dim3(kOut), dim3(256), 0, st, N, kOut, dO, grad + kOffB3);
         gemm(st, n, kH, kOut, dO, kOut, 0, params + kOffW3, kH, 0, A2, kH, 1.f, 0.f);
         hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * kH)), dim3(256), 0, st, N * kH, Z2, A2);
         // layer 2 (dZ2 in A2): dW2 = dZ2^T A1, db2, dA1 = dZ2 W2
-        gemm(st, kH, kH, n, A2, kH, 1, A1, kH, 0, grad + kOffW2, kH, 1.f, 0.f);
+        gemm(st, kH, kH, n, A2, kH, 1, A1, kH, 0, grad + kOffW2, kH, 1.f, 0.f, spl);
         hipLaunchKernelGGL(k_colsum, dim3(kH), dim3(256), 0, st, N, kH, A2, grad + kOffB2);
         gemm(st, n, kH, kH, A2, kH, 0, params + kOffW2, kH, 0, A1, kH, 1.f, 0.f);
         hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * kH)), dim3(256), 0, st, N * kH, Z1, A1);
         // layer 1 (dZ1 in A1): dW1 = dZ1^T X, db1
-        gemm(st, kH, kIn, n, A1, kH, 1, X, kIn, 0, grad + kOffW1, kIn, 1.f, 0.f);
+        gemm(st, kH, kIn, n, A1, kH, 1, X, kIn, 0, grad + kOffW1, kIn, 1.f, 0.f, spl);
         hipLaunchKernelGGL(k_colsum, dim3(kH), dim3(256), 0, st, N, kH, A1, grad + kOffB1);
     }
     hipError_t e = hipGetLastError();
