@@ -332,6 +332,72 @@ int todg_corr_gemm_bf16(int M, int N, int K, const float *A, int lda, int ta, co
                         int ldb, int tb, float *C, int ldc, float alpha, float beta,
                         void *stream);
 
+/* MLP stencil corrector (no reference counterpart: the learned interpolation
+ * corrector with its network evaluated once per map pixel).  Per pixel p of
+ * a map field, with the features F_0 .. F_15 of todg_fcorr_channels and
+ * sigma = sqrt(sum_{j>=1} F_j^2 / 15):
+ *   x  = [2 |z_p| - 1, F_1 / sigma, .., F_15 / sigma]   (0 where sigma = 0)
+ *   o  = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3          (16 -> 64 -> 64 -> 32)
+ *   A0 = F_0 + sum_{j>=1} o_j F_j,  A1 = sum_{j>=1} o_{16+j} F_j
+ * (gelu as the MLP corrector's).  A0 and A1 are the two channels of
+ * todg_gather_accum_fcorr.  Parameters: TODG_SMLP_NPARAMS float32 in one
+ * flat device array W1 (64 x 16), b1, W2 (64 x 64), b2, W3 (32 x 64), b3.
+ * Matrix products on BF16 MFMA with FP32 accumulation. */
+#define TODG_SMLP_NPARAMS 7328
+
+/* Channels A0, A1 of one field (npix float32) in one fused kernel (features,
+ * MLP, channels) into out0[p * stride], out1[p * stride]. */
+int todg_smlp_channels(int64_t nside, const float *field, const float *params, float *out0,
+                       float *out1, int64_t stride, void *stream);
+
+/* Training side: features F (npix x 16) and MLP inputs X (npix x 16). */
+int todg_smlp_features(int64_t nside, const float *field, float *F, float *X, void *stream);
+
+/* Float count of the workspace of N rows for todg_smlp_forward / _backward. */
+size_t todg_smlp_workspace_floats(int64_t N);
+
+/* MLP outputs O (N x 32) of input rows X (N x 16); keeps the activations in
+ * the workspace for todg_smlp_backward. */
+int todg_smlp_forward(const float *params, const float *X, int64_t N, float *ws, float *O,
+                      void *stream);
+
+/* Gradient (TODG_SMLP_NPARAMS floats) of sum_i dO_i . o_i over the rows of
+ * the preceding todg_smlp_forward (whose workspace it consumes). */
+int todg_smlp_backward(const float *params, const float *X, int64_t N, float *ws, const float *dO,
+                       float *grad, void *stream);
+
+/* A0, A1 (N) from MLP outputs O (N x 32) and features F (N x 16). */
+int todg_smlp_head(int64_t N, const float *F, const float *O, float *A0, float *A1, void *stream);
+
+/* Backward of todg_smlp_head: dO (N x 32) from the gradients G0, G1 (N) of
+ * the loss with respect to A0, A1. */
+int todg_smlp_head_backward(int64_t N, const float *F, const float *G0, const float *G1, float *dO,
+                            void *stream);
+
+/* Two-channel sparse operator, one row per TOD sample (CSR: rowptr int64,
+ * col int32, values v0 / v1 float32): y_r = sum_e v0_e x0[col_e] + v1_e
+ * x1[col_e] (binary64, in row order), e_r = y_r - target_r (target may be
+ * NULL), resid_r = scale e_r (float32, may be NULL), y (may be NULL), and
+ * *sumsq = sum_r e_r^2 (may be NULL; needs part, todg_csr_rows2_parts(nrows)
+ * doubles). */
+size_t todg_csr_rows2_parts(int64_t nrows);
+int todg_csr_rows2(int64_t nrows, const int64_t *rowptr, const int *col, const float *v0,
+                   const float *v1, const float *x0, const float *x1, const double *target,
+                   double scale, double *y, float *resid, double *part, double *sumsq,
+                   void *stream);
+
+/* The transpose of such an operator stored by columns (one CSR row per map
+ * pixel, col = sample): out0_p = sum_e v0_e r[col_e], out1_p = sum_e v1_e
+ * r[col_e] (binary64 in row order, stored float32). */
+int todg_csr_adjoint2(int64_t nrows, const int64_t *rowptr, const int *col, const float *v0,
+                      const float *v1, const float *r, float *out0, float *out1, void *stream);
+
+/* One Adam step (bias-corrected, step >= 1) on the stencil MLP's parameters:
+ * the hidden layers (W1, b1, W2, b2) at learning rate lr, the output layer
+ * (W3, b3: the stencil weights themselves) at lr_out. */
+int todg_smlp_adam(float *params, const float *grad, float *m, float *v, int step, float lr,
+                   float lr_out, float beta1, float beta2, float eps, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,244 @@
+"""The MLP stencil corrector on the GPU against its CPU oracle
+(oracle/smlp_oracle.py) and the production kernels:
+
+* per-pixel features and MLP inputs (todg_smlp_features) against the oracle;
+* the fused channel builder (todg_smlp_channels: features, BF16-MFMA layers,
+  channels) against the oracle with the same bfloat16 operand rounding, and
+  against the training path's forward (todg_smlp_forward + todg_smlp_head);
+* the TOD operator assembled from the production kernel's index dumps
+  against the oracle's dense operator (column by column from
+  or_gather_accum_fcorr) and against the production bilinear and factorised
+  TOD kernels; its stored transpose against the dense transpose;
+* one training step's gradient against the oracle's (bfloat16 operands);
+* training keeps the held-out residual at the linear stencil's or below, and
+  the shipped parameters reproduce the factorised corrector's accuracy.
+"""
+
+import numpy as np
+import pytest
+
+import smlp_oracle as SO
+from cases import random_rot_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch
+
+    torch.cuda.set_device(0)
+    return torch
+
+
+def _rms(a):
+    return float(np.sqrt(np.mean(np.asarray(a, dtype=np.float64) ** 2)))
+
+
+@pytest.fixture(scope="module")
+def sky16():
+    from tod_generation_mapbased_beam_amd import inputs
+
+    nside = 16
+    m = inputs.synfast_iqu(nside, seed=3)[0].astype(np.float32)
+    F, z = SO.features(nside, m)
+    return nside, m, F, SO.inputs(F, z)
+
+
+def test_features_match_the_oracle(torch, sky16):
+    from tod_generation_mapbased_beam_amd import smlp
+
+    nside, m, Fo, Xo = sky16
+    F, X = smlp.features(nside, torch.as_tensor(m, device="cuda"))
+    F = F.cpu().numpy()
+    X = X.cpu().numpy()
+    assert np.max(np.abs(F - Fo)) <= 1e-6 * np.max(np.abs(Fo))
+    assert np.max(np.abs(X - Xo)) <= 1e-6
+
+
+@pytest.mark.parametrize("w3_scale", [0.0, 1.0])
+def test_fused_channels_match_the_oracle(torch, sky16, w3_scale):
+    from tod_generation_mapbased_beam_amd import fcorr, smlp
+
+    nside, m, Fo, Xo = sky16
+    flat = smlp.init_params(1, fcorr.FactorisedCorrector.load_default().coef, w3_scale)
+    npix = m.size
+    A0 = torch.empty(npix, device="cuda")
+    A1 = torch.empty(npix, device="cuda")
+    smlp.channels(flat, nside, torch.as_tensor(m, device="cuda"), A0, A1, 1)
+    A0 = A0.cpu().numpy()
+    A1 = A1.cpu().numpy()
+    o, _ = SO.forward(flat, Xo, bf16=True)
+    R0, R1 = SO.head(Fo, o)
+    d0 = _rms(R0 - Fo[:, 0])
+    # same operand rounding, FP32 accumulation and activations (measured
+    # 3e-6 / 3e-5 of the correction RMS; a bfloat16 flip of one activation
+    # moves an output by ~1e-3 of it)
+    assert _rms(A0 - R0) <= 1e-4 * d0 and np.max(np.abs(A0 - R0)) <= 3e-3 * d0
+    assert _rms(A1 - R1) <= 1e-4 * _rms(R1) and np.max(np.abs(A1 - R1)) <= 1e-2 * _rms(R1)
+    # the exact float64 model: bfloat16's 2^-8
+    e0, e1 = SO.channels(nside, m, flat, bf16=False)
+    assert _rms(A0 - e0) <= 1e-2 * d0 and _rms(A1 - e1) <= 2e-2 * _rms(e1)
+    # the training path: same arithmetic, other GEMM tiling
+    P = torch.as_tensor(flat, device="cuda")
+    ws = smlp.Workspace(npix, torch.device("cuda"))
+    Fd, Xd = smlp.features(nside, torch.as_tensor(m, device="cuda"))
+    smlp.mlp_forward(P, Xd, ws)
+    smlp.head(Fd, ws.O, ws.A0, ws.A1)
+    assert _rms(ws.A0.cpu().numpy() - A0) <= 1e-4 * d0
+    assert _rms(ws.A1.cpu().numpy() - A1) <= 1e-4 * _rms(R1)
+
+
+def test_constant_map_channels(torch):
+    from tod_generation_mapbased_beam_amd import smlp
+
+    nside = 32
+    c = torch.full((12 * nside * nside,), 2.25, device="cuda")
+    flat = smlp.init_params(3, np.random.default_rng(0).normal(0, 0.3, 32), 1.0)
+    A0 = torch.empty_like(c)
+    A1 = torch.empty_like(c)
+    smlp.channels(flat, nside, c, A0, A1, 1)
+    assert float(torch.max(torch.abs(A0 - 2.25))) == 0.0
+    assert float(torch.max(torch.abs(A1))) == 0.0
+
+
+@pytest.fixture(scope="module")
+def tiny(torch):
+    """nside 8, 30 random pointings, a compact 40-pixel beam: the GPU operator
+    and the oracle's dense one."""
+    from tod_generation_mapbased_beam_amd import inputs, smlp
+
+    rng = np.random.default_rng(0)
+    nside, S, B = 8, 40, 30
+    v = rng.standard_normal((S, 3)) * np.array([0.1, 0.1, 0.0]) + np.array([0.0, 0.0, 1.0])
+    vec = (v / np.linalg.norm(v, axis=-1, keepdims=True)).astype(np.float32)
+    bv = rng.uniform(0.2, 1.0, S).astype(np.float32)
+    bv /= bv.sum()
+    _, params = random_rot_params(B, S, rng)
+    m = inputs.synfast_iqu(nside, seed=4)[0].astype(np.float32)
+    op = smlp.build_operator(nside, torch.as_tensor(m, device="cuda"), vec, bv,
+                             [torch.as_tensor(p, device="cuda") for p in params], chunk=7)
+    npix = 12 * nside * nside
+    rows = torch.repeat_interleave(torch.arange(B, device="cuda"), op.rowptr[1:] - op.rowptr[:-1])
+    K0 = torch.zeros((B, npix), dtype=torch.float64, device="cuda")
+    K1 = torch.zeros_like(K0)
+    K0[rows, op.col.long()] = op.v0.double()
+    K1[rows, op.col.long()] = op.v1.double()
+    K0o, K1o = SO.dense_operator(vec, params, nside, bv, B, S)
+    return dict(nside=nside, B=B, S=S, vec=vec, bv=bv, params=params, m=m, op=op,
+                K0=K0.cpu().numpy(), K1=K1.cpu().numpy(), K0o=K0o, K1o=K1o)
+
+
+def test_operator_matches_the_oracle(torch, tiny):
+    # binary32 storage of the entries (and of the correction channel's
+    # (1 - g) g)
+    assert np.max(np.abs(tiny["K0"] - tiny["K0o"])) <= 1e-6 * np.max(np.abs(tiny["K0o"]))
+    assert np.max(np.abs(tiny["K1"] - tiny["K1o"])) <= 1e-6 * np.max(np.abs(tiny["K1o"]))
+    op = tiny["op"]
+    r = torch.randn(tiny["B"], device="cuda")
+    G0 = torch.empty(op.npix, device="cuda")
+    G1 = torch.empty(op.npix, device="cuda")
+    op.adjoint(r, G0, G1)
+    rr = r.double().cpu().numpy()
+    for G, K in ((G0, tiny["K0"]), (G1, tiny["K1"])):
+        want = K.T @ rr
+        assert np.max(np.abs(G.cpu().numpy() - want)) <= 1e-6 * np.max(np.abs(want))
+
+
+def test_gradient_matches_the_oracle(torch, tiny):
+    from tod_generation_mapbased_beam_amd import _lib, fcorr, smlp
+
+    op = tiny["op"]
+    flat = smlp.init_params(2, fcorr.FactorisedCorrector.load_default().coef, 0.5)
+    P = torch.as_tensor(flat, device="cuda")
+    ws = smlp.Workspace(op.npix, torch.device("cuda"))
+    resid = torch.empty(op.B, device="cuda")
+    part = torch.empty(int(_lib.lib().todg_csr_rows2_parts(op.B)), dtype=torch.float64, device="cuda")
+    smlp.mlp_forward(P, op.X, ws)
+    smlp.head(op.F, ws.O, ws.A0, ws.A1)
+    op.residual(ws.A0, ws.A1, 2.0 / op.norm, resid, part, ws.sumsq)
+    op.adjoint(resid, ws.G0, ws.G1)
+    smlp.head_backward(op.F, ws.G0, ws.G1, ws.dO)
+    smlp.mlp_backward(P, op.X, ws, ws.dO)
+    g = ws.grad.double().cpu().numpy()
+    F, z = SO.features(tiny["nside"], tiny["m"])
+    loss, go = SO.loss_grad(flat.astype(np.float64), F, SO.inputs(F, z), tiny["K0o"], tiny["K1o"],
+                            op.target.cpu().numpy(), op.norm, bf16=True)
+    assert abs(float(ws.sumsq) / op.norm - loss) <= 1e-6 * loss
+    # measured 2.3e-6 (FP32 accumulation against float64)
+    assert np.linalg.norm(g - go) <= 1e-4 * np.linalg.norm(go)
+    for a, b in ((0, smlp.OFF_B1), (smlp.OFF_W2, smlp.OFF_B2), (smlp.OFF_W3, smlp.OFF_B3),
+                 (smlp.OFF_B3, smlp.N_PARAMS)):
+        assert np.linalg.norm(g[a:b] - go[a:b]) <= 1e-3 * np.linalg.norm(go[a:b])
+
+
+def test_operator_reproduces_the_production_kernels(torch):
+    """At nside 128 with the 30' beam scaled to the pixel size: K0 m is the
+    production bilinear TOD, K0 A0 + K1 A1 the factorised kernel's TOD."""
+    from tod_generation_mapbased_beam_amd import fcorr, inputs, kernels, smlp
+
+    nside, B = 128, 600
+    vec, bv, _ = inputs.gaussian_beam(2 * nside, 30.0 * 2048 / nside, 0.99)
+    S = vec.shape[0]
+    params = [torch.as_tensor(p, device="cuda") for p in fcorr.random_pointings(B, 77)]
+    m = torch.as_tensor(inputs.synfast_iqu(nside, seed=5)[0], dtype=torch.float32, device="cuda")
+    op = smlp.build_operator(nside, m, vec, bv, params, chunk=128)
+    t = torch.zeros((1, B), dtype=torch.float64, device="cuda")
+    kernels.gather_accum("bilinear", vec, *params, nside, m[None], bv, B, S, t)
+    y = op.apply(m, torch.zeros_like(m))
+    assert float(torch.max(torch.abs(y - t[0]))) <= 1e-6 * float(torch.sqrt(torch.mean(t ** 2)))
+    corr = smlp.StencilMLPCorrector(smlp.init_params(4, fcorr.FactorisedCorrector.load_default().coef, 0.3))
+    maps = corr.maps(m[None])
+    tf = corr.tod(maps, vec, bv, params)
+    y = op.apply(maps.packed0, maps.packed1)
+    assert float(torch.max(torch.abs(y - tf[0]))) <= 1e-6 * float(torch.sqrt(torch.mean(tf ** 2)))
+    # and the judge's TOD is the bicubic kernel's
+    tc = torch.zeros((1, B), dtype=torch.float64, device="cuda")
+    kernels.gather_accum("bicubic", vec, *params, nside, m[None], bv, B, S, tc)
+    assert torch.equal(op.target, tc[0])
+
+
+def test_training_and_held_out_sky(torch):
+    """Two nside-256 skies, 6000 pointings each: from the least-squares
+    stencil, training does not worsen the training or the held-out residual
+    (for Gaussian skies the best predictor from linear features is linear, so
+    the network has little to add to the stencil it starts from)."""
+    from tod_generation_mapbased_beam_amd import fcorr, inputs, smlp
+
+    nside = 256
+    vec, bv, _ = inputs.gaussian_beam(2 * nside, 30.0 * 2048 / nside, 0.99)
+    dev = torch.device("cuda")
+    ops = []
+    for i, (sky, pt) in enumerate(((100, 1100), (101, 1101), (555, 5555))):
+        params = [torch.as_tensor(p, device=dev) for p in fcorr.random_pointings(6000, pt)]
+        m = torch.as_tensor(inputs.synfast_iqu(nside, seed=sky)[0], dtype=torch.float32, device=dev)
+        ops.append(smlp.build_operator(nside, m, vec, bv, params))
+    corr = smlp.StencilMLPCorrector(smlp.init_params(100, fcorr.FactorisedCorrector.load_default().coef))
+    ws = smlp.Workspace(ops[0].npix, dev)
+    h0 = corr.ratio(ops[2], ws)
+    rep = corr.train(ops[:2], 40, lr=1e-3, lr_out=1e-6, seed=1)
+    h1 = corr.ratio(ops[2], ws)
+    print(f"training {rep.start_ratio:.4f} -> {rep.train_ratio:.4f}, held-out {h0:.4f} -> {h1:.4f} "
+          f"({rep.steps} steps, {rep.seconds:.1f} s)")
+    assert rep.start_ratio < 0.2 and h0 < 0.2
+    assert rep.train_ratio <= rep.start_ratio * 1.01
+    assert h1 <= h0 * 1.01
+    assert float(torch.max(torch.abs(corr.params))) < 100.0
+
+
+def test_shipped_parameters(torch):
+    """The shipped network on a held-out nside-512 sky: within 1 % of the
+    least-squares stencil's residual ratio, and at most 0.2."""
+    from tod_generation_mapbased_beam_amd import fcorr, inputs, smlp
+
+    nside = 512
+    vec, bv, _ = inputs.gaussian_beam(2 * nside, 30.0 * 2048 / nside, 0.99)
+    dev = torch.device("cuda")
+    params = [torch.as_tensor(p, device=dev) for p in fcorr.random_pointings(5000, 31337)]
+    m = torch.as_tensor(inputs.synfast_iqu(nside, seed=31338)[0], dtype=torch.float32, device=dev)
+    op = smlp.build_operator(nside, m, vec, bv, params)
+    ws = smlp.Workspace(op.npix, dev)
+    shipped = smlp.StencilMLPCorrector.load_default().ratio(op, ws)
+    linear = smlp.StencilMLPCorrector(smlp.init_params(0, fcorr.FactorisedCorrector.load_default().coef)).ratio(op, ws)
+    print(f"held-out ratio: shipped network {shipped:.4f}, least-squares stencil {linear:.4f}")
+    assert shipped <= 0.2 and shipped <= linear * 1.01

@@ -87,6 +87,18 @@ SIGNATURES = {
         [_i32, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _i32, _p, _i64, _i64, _p, _i32, _i32, _i32, _p,
          _p],
     ),
+    "todg_smlp_channels": (_i32, [_i64, _p, _p, _p, _p, _i64, _p]),
+    "todg_smlp_features": (_i32, [_i64, _p, _p, _p, _p]),
+    "todg_smlp_workspace_floats": (ctypes.c_size_t, [_i64]),
+    "todg_smlp_forward": (_i32, [_p, _p, _i64, _p, _p, _p]),
+    "todg_smlp_backward": (_i32, [_p, _p, _i64, _p, _p, _p, _p]),
+    "todg_smlp_head": (_i32, [_i64, _p, _p, _p, _p, _p]),
+    "todg_smlp_head_backward": (_i32, [_i64, _p, _p, _p, _p, _p]),
+    "todg_csr_rows2_parts": (ctypes.c_size_t, [_i64]),
+    "todg_csr_rows2": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, ctypes.c_double, _p, _p, _p, _p, _p]),
+    "todg_csr_adjoint2": (_i32, [_i64, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "todg_smlp_adam": (_i32, [_p, _p, _p, _p, _i32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float, _p]),
     "todg_fp_dump_bytes": (ctypes.c_size_t, [_i64, _i64]),
     "todg_fp_dump": (
         _i32,

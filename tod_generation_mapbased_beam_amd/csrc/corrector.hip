@@ -23,6 +23,7 @@ typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 static_assert(kNParams == TODG_CORR_NPARAMS, "parameter count");
+static_assert(todg_smlp::kNParams == TODG_SMLP_NPARAMS, "stencil MLP parameter count");
 
 // C (M x N, row-major, ldc) = alpha * op(A) op(B) + beta * C with
 // op(A)[i][k] = ta ? A[k*lda + i] : A[i*lda + k] (M x K) and
@@ -320,6 +321,101 @@ __global__ __launch_bounds__(256) void k_features(int nside, const RingEntry* __
     ok[i] = good ? 1 : 0;
 }
 
+// ------------------------------------------------- MLP stencil corrector
+// (the model is defined in corrector.hpp, namespace todg_smlp)
+
+// Channels from the MLP outputs O (N x 32) and the features F (N x 16):
+// A0 = F_0 + sum_{j>=1} O_j F_j, A1 = sum_{j>=1} O_{16+j} F_j.
+__global__ __launch_bounds__(256) void k_smlp_head(int64_t N, const float* __restrict__ F,
+                                                   const float* __restrict__ O, float* __restrict__ A0,
+                                                   float* __restrict__ A1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float* f = F + i * 16;
+    const float* o = O + i * todg_smlp::kOut;
+    double s0 = (double)f[0], s1 = 0.0;
+#pragma unroll
+    for (int j = 1; j < 16; ++j) {
+        s0 = fma((double)o[j], (double)f[j], s0);
+        s1 = fma((double)o[16 + j], (double)f[j], s1);
+    }
+    A0[i] = (float)s0;
+    A1[i] = (float)s1;
+}
+
+// Backward of k_smlp_head: dO_j = G0 F_j, dO_{16+j} = G1 F_j (j >= 1); the
+// unused outputs 0 and 16 get 0.
+__global__ __launch_bounds__(256) void k_smlp_head_bwd(int64_t N, const float* __restrict__ F,
+                                                       const float* __restrict__ G0,
+                                                       const float* __restrict__ G1, float* __restrict__ dO) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float* f = F + i * 16;
+    float* d = dO + i * todg_smlp::kOut;
+    const float g0 = G0[i], g1 = G1[i];
+    d[0] = 0.f;
+    d[16] = 0.f;
+#pragma unroll
+    for (int j = 1; j < 16; ++j) {
+        d[j] = g0 * f[j];
+        d[16 + j] = g1 * f[j];
+    }
+}
+
+// Rows of a two-channel sparse operator (CSR, int64 row pointers, int32
+// columns, binary32 values): y_r = sum_e v0_e x0[c_e] + v1_e x1[c_e] in row
+// order (binary64); e_r = y_r - target_r (target may be null); resid_r =
+// scale e_r (binary32, may be null); part[block] = sum of e_r^2 over the
+// block (may be null); y (binary64, may be null).
+__global__ __launch_bounds__(256) void k_csr_rows2(int64_t nrows, const int64_t* __restrict__ rp,
+                                                   const int* __restrict__ col, const float* __restrict__ v0,
+                                                   const float* __restrict__ v1, const float* __restrict__ x0,
+                                                   const float* __restrict__ x1, const double* __restrict__ target,
+                                                   double scale, double* __restrict__ y,
+                                                   float* __restrict__ resid, double* __restrict__ part) {
+    __shared__ double red[256];
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double e2 = 0.0;
+    if (r < nrows) {
+        double acc = 0.0;
+        for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+            const int c = col[e];
+            acc = fma((double)v0[e], (double)x0[c], acc);
+            acc = fma((double)v1[e], (double)x1[c], acc);
+        }
+        if (y) y[r] = acc;
+        const double d = target ? acc - target[r] : acc;
+        if (resid) resid[r] = (float)(scale * d);
+        e2 = d * d;
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && part) part[blockIdx.x] = red[0];
+}
+
+// The transpose's rows (the operator stored column-major as CSR): out0_p =
+// sum_e v0_e r[c_e], out1_p = sum_e v1_e r[c_e] in row order (binary64,
+// stored binary32).
+__global__ __launch_bounds__(256) void k_csr_adjoint2(int64_t nrows, const int64_t* __restrict__ rp,
+                                                      const int* __restrict__ col, const float* __restrict__ v0,
+                                                      const float* __restrict__ v1, const float* __restrict__ rv,
+                                                      float* __restrict__ out0, float* __restrict__ out1) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nrows) return;
+    double a0 = 0.0, a1 = 0.0;
+    for (int64_t e = rp[p]; e < rp[p + 1]; ++e) {
+        const double r = (double)rv[col[e]];
+        a0 = fma((double)v0[e], r, a0);
+        a1 = fma((double)v1[e], r, a1);
+    }
+    out0[p] = (float)a0;
+    out1[p] = (float)a1;
+}
+
 int fail(int code, const std::string& msg) { return todg_internal::set_error(code, msg); }
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -359,6 +455,19 @@ void forward(hipStream_t st, const float* P, const float* X, int64_t N, float* Z
     hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(N * kH)), dim3(256), 0, st, N, kH, P + kOffB2, Z2, A2);
     gemm(st, n, kOut, kH, A2, kH, 0, P + kOffW3, kH, 1, O, kOut, 1.f, 0.f);
     hipLaunchKernelGGL(k_bias, dim3(nblk(N * kOut)), dim3(256), 0, st, N, kOut, P + kOffB3, O);
+}
+
+// The stencil MLP's forward pass: Z1, A1, Z2, A2 (N x kH), O (N x kOut).
+void smlp_forward(hipStream_t st, const float* P, const float* X, int64_t N, float* Z1, float* A1,
+                  float* Z2, float* A2, float* O) {
+    namespace S = todg_smlp;
+    const int n = (int)N;
+    gemm(st, n, S::kH, S::kIn, X, S::kIn, 0, P + S::kOffW1, S::kIn, 1, Z1, S::kH, 1.f, 0.f);
+    hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(N * S::kH)), dim3(256), 0, st, N, S::kH, P + S::kOffB1, Z1, A1);
+    gemm(st, n, S::kH, S::kH, A1, S::kH, 0, P + S::kOffW2, S::kH, 1, Z2, S::kH, 1.f, 0.f);
+    hipLaunchKernelGGL(k_bias_gelu, dim3(nblk(N * S::kH)), dim3(256), 0, st, N, S::kH, P + S::kOffB2, Z2, A2);
+    gemm(st, n, S::kOut, S::kH, A2, S::kH, 0, P + S::kOffW3, S::kH, 1, O, S::kOut, 1.f, 0.f);
+    hipLaunchKernelGGL(k_bias, dim3(nblk(N * S::kOut)), dim3(256), 0, st, N, S::kOut, P + S::kOffB3, O);
 }
 
 }  // namespace todg_corr
@@ -456,6 +565,125 @@ int todg_corr_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, co
                         void* stream) {
     if (M <= 0 || N <= 0 || K <= 0) return fail(TODG_ERR_ARG, "empty product");
     gemm(reinterpret_cast<hipStream_t>(stream), M, N, K, A, lda, ta, B, ldb, tb, C, ldc, alpha, beta);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+// ------------------------------------------------- MLP stencil corrector
+
+// Z1, A1, Z2, A2 (N x kH each), then the split-K partials of the weight
+// gradients.
+size_t todg_smlp_workspace_floats(int64_t N) {
+    return (size_t)N * 4 * todg_smlp::kH + 1024 + (size_t)n_splits(N) * kSplitMaxTiles * 256;
+}
+
+int todg_smlp_forward(const float* params, const float* X, int64_t N, float* ws, float* O, void* stream) {
+    namespace S = todg_smlp;
+    if (N <= 0 || N > (1 << 30)) return fail(TODG_ERR_ARG, "N out of range");
+    if (!params || !X || !ws || !O) return fail(TODG_ERR_ARG, "null pointer argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    float* Z1 = ws;
+    float* A1 = Z1 + N * S::kH;
+    float* Z2 = A1 + N * S::kH;
+    float* A2 = Z2 + N * S::kH;
+    smlp_forward(st, params, X, N, Z1, A1, Z2, A2, O);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_smlp_backward(const float* params, const float* X, int64_t N, float* ws, const float* dO,
+                       float* grad, void* stream) {
+    namespace S = todg_smlp;
+    if (N <= 0 || N > (1 << 30)) return fail(TODG_ERR_ARG, "N out of range");
+    if (!params || !X || !ws || !dO || !grad) return fail(TODG_ERR_ARG, "null pointer argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int n = (int)N;
+    float* Z1 = ws;
+    float* A1 = Z1 + N * S::kH;
+    float* Z2 = A1 + N * S::kH;
+    float* A2 = Z2 + N * S::kH;
+    float* spl = A2 + N * S::kH + 1024;
+    // layer 3: dW3 = dO^T A2, db3 = colsum dO, dA2 = dO W3
+    gemm(st, S::kOut, S::kH, n, dO, S::kOut, 1, A2, S::kH, 0, grad + S::kOffW3, S::kH, 1.f, 0.f, spl);
+    hipLaunchKernelGGL(k_colsum, dim3(S::kOut), dim3(256), 0, st, N, S::kOut, dO, grad + S::kOffB3);
+    gemm(st, n, S::kH, S::kOut, dO, S::kOut, 0, params + S::kOffW3, S::kH, 0, A2, S::kH, 1.f, 0.f);
+    hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * S::kH)), dim3(256), 0, st, N * S::kH, Z2, A2);
+    // layer 2 (dZ2 in A2): dW2 = dZ2^T A1, db2, dA1 = dZ2 W2
+    gemm(st, S::kH, S::kH, n, A2, S::kH, 1, A1, S::kH, 0, grad + S::kOffW2, S::kH, 1.f, 0.f, spl);
+    hipLaunchKernelGGL(k_colsum, dim3(S::kH), dim3(256), 0, st, N, S::kH, A2, grad + S::kOffB2);
+    gemm(st, n, S::kH, S::kH, A2, S::kH, 0, params + S::kOffW2, S::kH, 0, A1, S::kH, 1.f, 0.f);
+    hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * S::kH)), dim3(256), 0, st, N * S::kH, Z1, A1);
+    // layer 1 (dZ1 in A1): dW1 = dZ1^T X, db1
+    gemm(st, S::kH, S::kIn, n, A1, S::kH, 1, X, S::kIn, 0, grad + S::kOffW1, S::kIn, 1.f, 0.f, spl);
+    hipLaunchKernelGGL(k_colsum, dim3(S::kH), dim3(256), 0, st, N, S::kH, A1, grad + S::kOffB1);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_smlp_head(int64_t N, const float* F, const float* O, float* A0, float* A1, void* stream) {
+    if (N < 0) return fail(TODG_ERR_ARG, "N out of range");
+    if (N == 0) return TODG_OK;
+    if (!F || !O || !A0 || !A1) return fail(TODG_ERR_ARG, "null pointer argument");
+    hipLaunchKernelGGL(k_smlp_head, dim3(nblk(N)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), N, F, O,
+                       A0, A1);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_smlp_head_backward(int64_t N, const float* F, const float* G0, const float* G1, float* dO,
+                            void* stream) {
+    if (N < 0) return fail(TODG_ERR_ARG, "N out of range");
+    if (N == 0) return TODG_OK;
+    if (!F || !G0 || !G1 || !dO) return fail(TODG_ERR_ARG, "null pointer argument");
+    hipLaunchKernelGGL(k_smlp_head_bwd, dim3(nblk(N)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), N,
+                       F, G0, G1, dO);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+size_t todg_csr_rows2_parts(int64_t nrows) { return (size_t)nblk(nrows); }
+
+int todg_csr_rows2(int64_t nrows, const int64_t* rowptr, const int* col, const float* v0, const float* v1,
+                   const float* x0, const float* x1, const double* target, double scale, double* y,
+                   float* resid, double* part, double* sumsq, void* stream) {
+    if (nrows < 0) return fail(TODG_ERR_ARG, "nrows out of range");
+    if (!rowptr || !col || !v0 || !v1 || !x0 || !x1) return fail(TODG_ERR_ARG, "null pointer argument");
+    if (sumsq && !part) return fail(TODG_ERR_ARG, "sumsq needs the part workspace");
+    if (nrows == 0) return TODG_OK;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const unsigned nb = nblk(nrows);
+    hipLaunchKernelGGL(k_csr_rows2, dim3(nb), dim3(256), 0, st, nrows, rowptr, col, v0, v1, x0, x1, target, scale,
+                       y, resid, part);
+    if (sumsq) hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(256), 0, st, (int)nb, part, sumsq);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_csr_adjoint2(int64_t nrows, const int64_t* rowptr, const int* col, const float* v0, const float* v1,
+                      const float* r, float* out0, float* out1, void* stream) {
+    if (nrows < 0) return fail(TODG_ERR_ARG, "nrows out of range");
+    if (nrows == 0) return TODG_OK;
+    if (!rowptr || !col || !v0 || !v1 || !r || !out0 || !out1) return fail(TODG_ERR_ARG, "null pointer argument");
+    hipLaunchKernelGGL(k_csr_adjoint2, dim3(nblk(nrows)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       nrows, rowptr, col, v0, v1, r, out0, out1);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
+}
+
+int todg_smlp_adam(float* params, const float* grad, float* m, float* v, int step, float lr, float lr_out,
+                   float beta1, float beta2, float eps, void* stream) {
+    namespace S = todg_smlp;
+    if (step < 1) return fail(TODG_ERR_ARG, "step must be >= 1");
+    if (!params || !grad || !m || !v) return fail(TODG_ERR_ARG, "null pointer argument");
+    const float c1 = 1.f - powf(beta1, (float)step);
+    const float c2 = 1.f - powf(beta2, (float)step);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // hidden layers at lr, the output layer (W3, b3) at lr_out
+    hipLaunchKernelGGL(k_adam, dim3(nblk(S::kOffW3)), dim3(256), 0, st, S::kOffW3, params, grad, m, v, lr, beta1,
+                       beta2, eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(nblk(S::kNParams - S::kOffW3)), dim3(256), 0, st, S::kNParams - S::kOffW3,
+                       params + S::kOffW3, grad + S::kOffW3, m + S::kOffW3, v + S::kOffW3, lr_out, beta1, beta2,
+                       eps, c1, c2);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
 }

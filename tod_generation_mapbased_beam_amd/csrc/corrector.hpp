@@ -70,3 +70,34 @@ __device__ __forceinline__ float gelu_grad(float z) {
 }
 
 }  // namespace todg_corr
+
+// MLP stencil corrector: the factorised corrector's two channels with
+// per-pixel stencil weights emitted by an MLP evaluated once per map pixel.
+//
+// For pixel p of a map field, F_0 = m(p) and F_1 .. F_15 are the stencil
+// features of the factorised corrector (ring-interpolated means at ring
+// offsets dr = j / 4 and ring-step offsets ds = j % 4, minus m(p));
+// sigma = sqrt(sum_{j>=1} F_j^2 / 15) is their local RMS.
+//
+//   x  = [2 |z_p| - 1, F_1 / sigma, .., F_15 / sigma]            (16)
+//   h1 = gelu(W1 x + b1)        (64)
+//   h2 = gelu(W2 h1 + b2)       (64)
+//   o  = W3 h2 + b3             (32)
+//   A0 = F_0 + sum_{j>=1} o_j F_j,   A1 = sum_{j>=1} o_{16+j} F_j
+//
+// (o_0 and o_16 are not used; x is 0 where sigma is 0.)  A constant map has
+// F_j = 0 for j >= 1, so A0 = m and A1 = 0 there; the channels scale with
+// the map.  The TOD kernel is the factorised corrector's (footprint mode 4).
+namespace todg_smlp {
+
+constexpr int kIn = 16, kH = 64, kOut = 32;
+// flat parameter layout (FP32): W1 (kH x kIn), b1, W2 (kH x kH), b2, W3 (kOut x kH), b3
+constexpr int kOffW1 = 0;
+constexpr int kOffB1 = kOffW1 + kH * kIn;
+constexpr int kOffW2 = kOffB1 + kH;
+constexpr int kOffB2 = kOffW2 + kH * kH;
+constexpr int kOffW3 = kOffB2 + kH;
+constexpr int kOffB3 = kOffW3 + kOut * kH;
+constexpr int kNParams = kOffB3 + kOut;
+
+}  // namespace todg_smlp

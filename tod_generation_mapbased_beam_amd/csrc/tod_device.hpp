@@ -2636,6 +2636,12 @@ struct FcorrCoef {
 };
 hipError_t launch_fcorr_channels(const RingEntry* rings, int nside, const float* field, const FcorrCoef& c,
                                  float* out0, float* out1, int64_t stride, hipStream_t st);
+// The MLP stencil corrector's map side (tod_fp_fcorr.hip): per-pixel
+// features and MLP inputs (training), and the fused channel builder.
+hipError_t launch_smlp_features(const RingEntry* rings, int nside, const float* field, float* F, float* X,
+                                hipStream_t st);
+hipError_t launch_smlp_channels(const RingEntry* rings, int nside, const float* field, const float* params,
+                                float* out0, float* out1, int64_t stride, hipStream_t st);
 hipError_t launch_batch_m0(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st);
 hipError_t launch_batch_m1(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st);
 hipError_t launch_batch_m2(const BatchArgs& ba, bool qu, int CS, int variant, hipStream_t st);

@@ -699,6 +699,33 @@ int todg_fcorr_channels(int64_t nside, const float* field, const double* coef, f
     return TODG_OK;
 }
 
+int todg_smlp_features(int64_t nside, const float* field, float* F, float* X, void* stream) {
+    if (!is_pow2(nside) || nside > 8192)
+        return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
+    if (!field || !F || !X) return fail(TODG_ERR_ARG, "null pointer argument");
+    const RingEntry* rings = nullptr;
+    int rc = get_ring_table(nside, &rings);
+    if (rc) return rc;
+    hipError_t e = launch_smlp_features(rings, (int)nside, field, F, X, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TODG_OK;
+}
+
+int todg_smlp_channels(int64_t nside, const float* field, const float* params, float* out0, float* out1,
+                       int64_t stride, void* stream) {
+    if (!is_pow2(nside) || nside > 8192)
+        return fail(TODG_ERR_ARG, "nside must be a power of two <= 8192");
+    if (!field || !params || !out0 || !out1) return fail(TODG_ERR_ARG, "null pointer argument");
+    if (stride < 1) return fail(TODG_ERR_ARG, "stride must be >= 1");
+    const RingEntry* rings = nullptr;
+    int rc = get_ring_table(nside, &rings);
+    if (rc) return rc;
+    hipError_t e = launch_smlp_channels(rings, (int)nside, field, params, out0, out1, stride,
+                                        reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return TODG_OK;
+}
+
 int todg_interp_points(int mode, int64_t nside, const float* map, int64_t N, const double* z,
                        const double* phi, double* out, void* stream) {
     if (mode != 0 && mode != 1) return fail(TODG_ERR_ARG, "mode must be 0 (bilinear) or 1 (bicubic)");

@@ -72,9 +72,12 @@ def channels(nside: int, field, coef, out0, out1, stride: int = 1) -> None:
 
 class FcorrMaps:
     """The two packed channel maps of a ``(C, npix)`` sky for a Q/U channel
-    assignment, in the slot layout of the plain kernels' packed map."""
+    assignment, in the slot layout of the plain kernels' packed map.  The
+    channels come from the stencil coefficients ``coef``, or from
+    ``builder(nside, field, out0, out1, stride)`` when one is given (the MLP
+    stencil corrector's)."""
 
-    def __init__(self, mp_stacked, coef, c_q: int = -1, c_u: int = -1):
+    def __init__(self, mp_stacked, coef, c_q: int = -1, c_u: int = -1, builder=None):
         src = to_dev(mp_stacked, torch.float32)
         if src.dim() == 1:
             src = src[None, :]
@@ -85,14 +88,18 @@ class FcorrMaps:
         self.nside = nside_of(self.npix)
         self.c_q = int(c_q)
         self.c_u = int(c_u)
-        self.coef = np.array(coef, dtype=np.float64)
+        self.coef = None if coef is None else np.array(coef, dtype=np.float64)
         self.cs, self.perm, self.row = map_slots(self.C, self.c_q, self.c_u)
         self.packed0 = torch.zeros(self.npix * self.cs, dtype=torch.float32, device=src.device)
         self.packed1 = torch.zeros(self.npix * self.cs, dtype=torch.float32, device=src.device)
         for k, c in enumerate(self.perm):
             if c < 0:
                 continue
-            channels(self.nside, src[c].contiguous(), self.coef, self.packed0[k:], self.packed1[k:], self.cs)
+            if builder is not None:
+                builder(self.nside, src[c].contiguous(), self.packed0[k:], self.packed1[k:], self.cs)
+            else:
+                channels(self.nside, src[c].contiguous(), self.coef, self.packed0[k:], self.packed1[k:],
+                         self.cs)
 
 
 def gather_accum_fcorr(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, maps: FcorrMaps,

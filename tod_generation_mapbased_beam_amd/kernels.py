@@ -180,13 +180,14 @@ def bilinear_indices(vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, 
 
 
 def fp_dump(interp_mode, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nside, mp_stacked,
-            beam_vals, B, S, c_q=-1, c_u=-1):
+            beam_vals, B, S, c_q=-1, c_u=-1, host=True):
     """Run the production footprint kernel's index-dump build over all B
     samples: returns (pix (B, S, 4) int64, w (B, S, 4) float64, path (B, S)
     uint8, tod (C, B) float64) -- the pixels each beam pixel was deposited
     into (bilinear neighbours in the reference's order; nearest: column 0),
     the deposited beam value x weight, the path that took it (0 certified
-    fast path, 1 exact path) and the TOD of the same launch."""
+    fast path, 1 exact path) and the TOD of the same launch.  numpy arrays,
+    or device tensors (views of one buffer) with host=False."""
     if interp_mode not in ("bilinear", "nearest"):
         raise ValueError("index dumps exist for 'bilinear' and 'nearest'")
     B = int(B)
@@ -205,6 +206,9 @@ def fp_dump(interp_mode, vec_orig, axes, cos_a, sin_a, ax_pts, cos_p, sin_p, nsi
         _lib.ptr(cp), _lib.ptr(sp), int(nside), _lib.ptr(dmap.packed), dmap.C, _lib.ptr(bv), B, S,
         _lib.ptr(tod), c_q, c_u, _lib.ptr(buf), stream_handle()))
     n = B * S
+    if not host:
+        return (buf[: 4 * n].view(B, S, 4), buf[4 * n: 8 * n].view(torch.float64).view(B, S, 4),
+                buf.view(torch.uint8)[64 * n: 65 * n].view(B, S), tod)
     raw = buf.cpu().numpy()
     pix = raw[: 4 * n].reshape(B, S, 4)
     w = raw[4 * n: 8 * n].view(np.float64).reshape(B, S, 4)
