@@ -43,9 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one wave64 VALU
 # instruction per SIMD every 4 cycles (16 lanes; binary32 and binary64 alike)
 VALU_PEAK_GINST = 1024 * 2.4 / 4.0
-# neighbours x 3 fields x float32 (learned: both channels of the factorised
-# corrector; learned-mlp: the MLP corrector's 4 x 4 neighbourhood)
-BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 96, "learned-mlp": 192}
+# neighbours x 3 fields x float32 (learned, learned-linear: both channels of
+# the factorised gather; learned-mlp: the MLP corrector's 4 x 4 neighbourhood)
+BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 96, "learned-linear": 96, "learned-mlp": 192}
 DTYPE = "mixed: f64 rotation, index path and accumulation; f32 map, weights and near-boresight spin-2"
 
 
@@ -178,12 +178,15 @@ def main():
     ap.add_argument("--fwhm", type=float, default=30.0)
     ap.add_argument("--samples", type=int, default=1728000)
     ap.add_argument("--fs", type=float, default=20.0)
-    ap.add_argument("--mode", choices=["bilinear", "nearest", "learned", "learned-mlp"], default="bilinear",
-                    help="learned: the factorised corrector with the package's fitted coefficients (its "
-                         "channel maps are built before timing, like the packed map; the line adds the "
-                         "plain kernel's time on the same workload and the residual against the bicubic "
-                         "judge on a synthetic sky); learned-mlp: the per-beam-pixel MLP corrector with "
-                         "seeded random parameters (its work does not depend on their values), bound 4")
+    ap.add_argument("--mode", choices=["bilinear", "nearest", "learned", "learned-linear", "learned-mlp"],
+                    default="bilinear",
+                    help="learned: the MLP stencil corrector with the package's trained parameters (its "
+                         "network builds the two channel maps once per sky, before timing, like the packed "
+                         "map; the line adds the plain kernel's time on the same workload, the channel "
+                         "build and the residual against the bicubic judge on a synthetic sky); "
+                         "learned-linear: the same gather with the least-squares stencil's channels; "
+                         "learned-mlp: the per-beam-pixel MLP corrector with seeded random parameters (its "
+                         "work does not depend on their values), bound 4")
     ap.add_argument("--judge-samples", type=int, default=100_000,
                     help="learned: samples of the day (evenly spaced) on which the residual against the "
                          "bicubic judge is measured on a synthetic nside-2048 sky; 0 skips it")
@@ -234,7 +237,7 @@ def main():
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
     variant, variant_note = args.variant, "given"
     if variant < 0:
-        if args.mode in ("learned", "learned-mlp"):
+        if args.mode.startswith("learned"):
             variant, variant_note = 0, "the corrector has one build"
         else:
             from tod_generation_mapbased_beam_amd import autotune
@@ -245,10 +248,14 @@ def main():
                 f"{k}: {v:.2f} ms" for k, v in sorted(tms.items()))
             log(f"autotune: variant {variant} ({variant_note})")
     prep_ms = None
-    if args.mode == "learned":
-        from tod_generation_mapbased_beam_amd import fcorr
+    if args.mode in ("learned", "learned-linear"):
+        from tod_generation_mapbased_beam_amd import fcorr, smlp
 
-        fc = fcorr.FactorisedCorrector.load_default()
+        if args.mode == "learned":
+            fc = smlp.StencilMLPCorrector.load_default()
+        else:
+            fc = fcorr.FactorisedCorrector.load_default()
+        fmaps = fc.maps(mp, 1, 2)  # (first call: library and ring table set-up)
         torch.cuda.synchronize()
         t_prep = time.perf_counter()
         fmaps = fc.maps(mp, 1, 2)
@@ -294,14 +301,16 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     work = float(B) * S
     value = work / (ms_per_step / 1e3)
-    roof = roofline(f"single {args.mode} nside {args.nside} S {S} B {B}", kern_ms, work,
+    # (learned and learned-linear run the same kernel on the same workload)
+    roof_mode = "learned" if args.mode == "learned-linear" else args.mode
+    roof = roofline(f"single {roof_mode} nside {args.nside} S {S} B {B}", kern_ms, work,
                     BYTES_PER_BPS[args.mode], variant)
     log(f"timed {args.steps} steps: {ms_per_step:.2f} ms/step, kernel {kern_ms:.2f} ms, "
         f"{value:.3e} bp.s/s; VALU issue {roof['frac']}, HBM {roof['hbm_frac']}")
     finite = bool(torch.isfinite(tod).all())
 
     learned = None
-    if args.mode == "learned":
+    if args.mode in ("learned", "learned-linear"):
         learned = learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms,
                                  prep_ms, B, S)
     cpu = None
@@ -345,13 +354,14 @@ def main():
 
 
 def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms, prep_ms, B, S):
-    """The factorised corrector's line: the plain bilinear kernel timed on the
-    same workload (the autotuner's variant), the time ratio with and without
-    the one-off channel build, and the RMS / max residual against the bicubic
-    judge on a synthetic sky, relative to bilinear's."""
+    """The corrector's line (the MLP stencil corrector, or the least-squares
+    stencil's channels): the plain bilinear kernel timed on the same workload
+    (the autotuner's variant), the time ratio with and without the one-off
+    channel build, and the RMS / max residual against the bicubic judge on a
+    synthetic sky, relative to bilinear's."""
     import torch
 
-    from tod_generation_mapbased_beam_amd import autotune, fcorr, inputs, kernels
+    from tod_generation_mapbased_beam_amd import autotune, fcorr, inputs, kernels, smlp
 
     pv, _ = autotune.tune_single(dvec, dbv, dparams, dmap, mode="bilinear", max_samples=131072, reps=3)
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
@@ -365,11 +375,18 @@ def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fma
         if i:
             ts.append(e0.elapsed_time(e1))
     plain_ms = float(np.mean(ts))
-    out = {"method": "factorised corrector (todg_fcorr_channels + todg_gather_accum_fcorr)",
+    if args.mode == "learned":
+        method = ("MLP stencil corrector: 16-64-64-32 GELU network per map pixel on BF16 MFMA "
+                  "(todg_smlp_channels) + todg_gather_accum_fcorr")
+        path = smlp.DEFAULT_PATH
+    else:
+        method = "least-squares stencil channels (todg_fcorr_channels) + todg_gather_accum_fcorr"
+        path = fcorr.DEFAULT_PATH
+    out = {"method": method,
            "plain_bilinear_kernel_ms": plain_ms, "plain_variant": int(pv), "kernel_ms": kern_ms,
            "time_ratio_over_plain": kern_ms / plain_ms, "channel_build_ms": prep_ms,
            "time_ratio_with_channel_build": (kern_ms + prep_ms) / plain_ms,
-           "coefficients": os.path.relpath(fcorr.DEFAULT_PATH, REPO)}
+           "parameters": os.path.relpath(path, REPO)}
     log(f"learned: kernel {kern_ms:.2f} ms vs plain {plain_ms:.2f} ms (variant {pv}): "
         f"{kern_ms / plain_ms:.3f}x; with the channel build {(kern_ms + prep_ms) / plain_ms:.3f}x")
     if args.judge_samples > 0:

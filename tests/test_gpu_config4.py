@@ -7,7 +7,10 @@ against the plain bilinear kernel and the bicubic judge on the same GPU.
 Checks: the coefficients fitted here on 8 nside-512 skies (and the ones the
 package ships) bring the RMS TOD residual against the judge to <= 0.20 of
 bilinear's on every field, on a held-out nside-512 sky and on the nside-2048
-production sky; the corrected kernel takes <= 1.5x the plain kernel's time;
+production sky, and so do the channels of the shipped MLP stencil corrector
+(the network evaluated per map pixel on BF16 MFMA); the corrected kernel
+takes <= 1.5x the plain kernel's time, also with the MLP's channel build
+for the sky counted once against the 8 detectors;
 and the corrected TOD is steadier than bilinear's under rolls of the
 circular beam and under rotations of the sky about the polar axis (both
 leave a perfect interpolator's TOD unchanged).
@@ -94,30 +97,40 @@ def test_fit_and_held_out_sky(torch, fitted):
 
 def test_production_residual_and_time(torch, fitted, sky):
     """8 detectors x 1.728M samples: RMS ratio per field and kernel times."""
-    from tod_generation_mapbased_beam_amd import inputs, kernels as K
+    from tod_generation_mapbased_beam_amd import fcorr as F, inputs, kernels as K
     from tod_generation_mapbased_beam_amd import tod_rotations as R
     from tod_generation_mapbased_beam_amd.device import DeviceMap
+
+    from tod_generation_mapbased_beam_amd import smlp
 
     corr, _ = fitted
     v, b, S = _beam(torch, 2 * NSIDE, 30.0)
     dmap = DeviceMap(sky, 1, 2)
     maps = corr.maps(sky, 1, 2)
-    ss = {k: torch.zeros(3, dtype=torch.float64, device="cuda") for k in ("bil", "fc")}
-    mx = {k: torch.zeros(3, dtype=torch.float64, device="cuda") for k in ("bil", "fc")}
-    t_ms = {"bil": 0.0, "fc": 0.0}
+    net = smlp.StencilMLPCorrector.load_default()
+    net.maps(sky, 1, 2)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    nmaps = net.maps(sky, 1, 2)
+    e1.record()
+    torch.cuda.synchronize()
+    build_ms = e0.elapsed_time(e1)
+    ss = {k: torch.zeros(3, dtype=torch.float64, device="cuda") for k in ("bil", "fc", "ml")}
+    mx = {k: torch.zeros(3, dtype=torch.float64, device="cuda") for k in ("bil", "fc", "ml")}
+    t_ms = {"bil": 0.0, "fc": 0.0, "ml": 0.0}
     for d in range(N_DET):
         th, ph, ps = inputs.ring_scan(B, 20.0, t0=3600.0 * d)
         params = [torch.as_tensor(p, device="cuda")
                   for p in R.rotation_params_for_pointing(th, ph, ps + 0.37 * d)]
         tb, tf, tc = _tods(torch, NSIDE, dmap, maps, v, b, S, params, B)
-        for k, t in (("bil", tb), ("fc", tf)):
+        tm = torch.zeros_like(tf)
+        F.gather_accum_fcorr(v, *params, NSIDE, nmaps, b, B, S, tm)
+        for k, t in (("bil", tb), ("fc", tf), ("ml", tm)):
             dd = t - tc
             ss[k] += torch.sum(dd * dd, dim=1)
             mx[k] = torch.maximum(mx[k], torch.max(torch.abs(dd), dim=1).values)
         # kernel times: best of two launches each
-        from tod_generation_mapbased_beam_amd import fcorr as F
-
-        for k in ("bil", "fc"):
+        for k in ("bil", "fc", "ml"):
             best = math.inf
             for _ in range(2):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -125,20 +138,29 @@ def test_production_residual_and_time(torch, fitted, sky):
                 if k == "bil":
                     K._gather_accum_fused_jit(v, *params, NSIDE, dmap, b, B, S, tb, 1, 2, variant=0)
                 else:
-                    F.gather_accum_fcorr(v, *params, NSIDE, maps, b, B, S, tf)
+                    F.gather_accum_fcorr(v, *params, NSIDE, maps if k == "fc" else nmaps, b, B, S, tf)
                 e1.record()
                 torch.cuda.synchronize()
                 best = min(best, e0.elapsed_time(e1))
             t_ms[k] += best
-        print(f"detector {d}: bilinear {t_ms['bil']:.1f} ms, corrected {t_ms['fc']:.1f} ms (cumulative)",
-              flush=True)
+        print(f"detector {d}: bilinear {t_ms['bil']:.1f} ms, corrected {t_ms['fc']:.1f} ms, MLP stencil "
+              f"{t_ms['ml']:.1f} ms (cumulative)", flush=True)
     rms = (torch.sqrt(ss["fc"] / ss["bil"])).tolist()
     mxr = (mx["fc"] / mx["bil"]).tolist()
     ratio = t_ms["fc"] / t_ms["bil"]
     print(f"8 x 1.728M: RMS residual ratio (rows I, Q, U) {np.round(rms, 4).tolist()}, max ratio "
           f"{np.round(mxr, 4).tolist()}; time {t_ms['fc']:.1f} / {t_ms['bil']:.1f} ms = {ratio:.3f}x")
+    rms_ml = (torch.sqrt(ss["ml"] / ss["bil"])).tolist()
+    mxr_ml = (mx["ml"] / mx["bil"]).tolist()
+    ratio_ml = t_ms["ml"] / t_ms["bil"]
+    ratio_build = (t_ms["ml"] + build_ms) / t_ms["bil"]
+    print(f"MLP stencil corrector: RMS residual ratio {np.round(rms_ml, 4).tolist()}, max ratio "
+          f"{np.round(mxr_ml, 4).tolist()}; time {ratio_ml:.3f}x, with the sky's channel build "
+          f"({build_ms:.1f} ms) {ratio_build:.3f}x")
     assert max(rms) <= 0.20
     assert ratio <= 1.5
+    assert max(rms_ml) <= 0.20
+    assert ratio_ml <= 1.5 and ratio_build <= 1.5
 
 
 def test_roll_stability(torch, fitted, sky):

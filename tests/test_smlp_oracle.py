@@ -20,7 +20,6 @@ import oracle as O
 import smlp_oracle as SO
 
 
-
 def _tiny(seed=0, nside=4, B=6, S=24):
     rng = np.random.default_rng(seed)
     v = rng.standard_normal((S, 3)) * np.array([0.12, 0.12, 0.0]) + np.array([0.0, 0.0, 1.0])
