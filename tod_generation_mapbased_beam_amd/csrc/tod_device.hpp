@@ -11,6 +11,7 @@
 #include <math.h>
 
 #include <mutex>
+#include <type_traits>
 
 #include "healpix_ring.hpp"
 #include "internal.hpp"
@@ -1402,6 +1403,7 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
 // it has laid out the next sample.
 constexpr int kClaim = 256;  // beam pixels per claim (4 per lane)
 constexpr int kClaimSub = 2;  // consecutive beam pixels per lane in a claim's block
+constexpr int kTailPixels = 1024;  // the beam's last >= 1024 pixels go in one-block claims
 static_assert((kClaim / 64) % kClaimSub == 0, "whole blocks per claim");
 
 // A claim is kClaim / (64 kClaimSub) blocks of 64 kClaimSub beam pixels; in
@@ -1456,15 +1458,22 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
                                                        dd, dbase);
     }
     // claims are pipelined: the next chunk is claimed, and its first beam
-    // pixel fetched, while the current chunk is swept
+    // pixel fetched, while the current chunk is swept.  Claims 0 .. nbig - 1
+    // hold CLAIM beam pixels each; the last kTailPixels or more of the beam
+    // go in claims of one 64 kClaimSub block, so that the waves reach the
+    // barrier after phase 1 within a block's sweep of each other.
+    constexpr int kTail = 64 * kClaimSub;
+    const int nbig = S > kTailPixels + CLAIM ? (S - kTailPixels) / CLAIM : 0;
     auto claim = [&]() {
         int c = 0;
         if (lane == 0) c = atomicAdd(&J->claim, 1);
-        return __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64)) * CLAIM;
+        c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, 64));
+        return c < nbig ? c * CLAIM : nbig * CLAIM + (c - nbig) * kTail;
     };
     int s0 = claim();
     float nv0 = 0.f, nv1 = 0.f, nv2 = 0.f, nbv = 0.f;
     constexpr int kPer = CLAIM / 64;
+    constexpr int kPerTail = kTail / 64;
     constexpr int kSub = kClaimSub;
     auto in_claim = [](int it) { return claim_offset(it); };
     // beam pixel sb + in_claim(it) + kSub lane of a claim starting at sb
@@ -1481,18 +1490,20 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
         nv2 = vp[2];
         nbv = *(gptr<const float>)((gptr<const char>)bvs + o4);
     };
-    if (S > 0) fetch(s0, 0);
-    while (s0 < S) {
+    // one claim of NPER iterations starting at s0 (uniform); returns the
+    // next claim's start, whose first beam pixel it has fetched
+    auto sweep = [&](auto nper) {
+        constexpr int NPER = decltype(nper)::value;
         const int s1 = claim();
         uint32_t slow = 0;
 #pragma unroll
-        for (int it = 0; it < kPer; ++it) {
+        for (int it = 0; it < NPER; ++it) {
             const int s = s0 + in_claim(it) + lane * kSub;
             const double v0 = (double)nv0, v1 = (double)nv1, v2 = (double)nv2;
             const float bvf = nbv;
             // fetch the next beam pixel early (the next chunk's first one
             // after the last iteration)
-            if (it + 1 < kPer)
+            if (it + 1 < NPER)
                 fetch(s0, it + 1);
             else
                 fetch(s1, 0);
@@ -1518,8 +1529,13 @@ __device__ __forceinline__ Acc4 fp_phase1(const TodArgs& a, FpJob* J, const FpFr
 #pragma unroll
             for (int j = 0; j < kMaxSlots; ++j) accd.v[j] += d.v[j];
         }
-        s0 = s1;
-    }
+        return s1;
+    };
+    if (S > 0) fetch(s0, 0);
+    // (a big claim starts below nbig CLAIM <= S - kTailPixels, a tail claim
+    // at or above it)
+    while (s0 < nbig * CLAIM) s0 = sweep(std::integral_constant<int, kPer>{});
+    while (s0 < S) s0 = sweep(std::integral_constant<int, kPerTail>{});
     return accd;
 }
 
