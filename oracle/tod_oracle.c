@@ -14,9 +14,16 @@
  *   - int(x)  : truncation toward zero;
  *   - max(-1, min(1, v)) : CPython's first-wins comparison order.
  *
- * The reference's per-sample spin-2 cache (tod_bilinear.py:151-199) is a
- * memo keyed by pixel index whose stored values are a deterministic function
- * of that pixel, so it is left out: recomputing gives the same bits.
+ * The reference's per-sample spin-2 cache (tod_bilinear.py:151-199) is keyed
+ * by pixel index, and the stored factors are a deterministic function of the
+ * neighbour's (z, phi), which the pixel index fixes -- except in the float-mod
+ * corner where the slot coordinate rounds to exactly n (see py_fmod below):
+ * the out-of-ring index first + n is then the next ring's first pixel, whose
+ * cached factors (if that pixel was met first in the sample) the reference
+ * would return for a neighbour that carries the current ring's (z, phi).
+ * The cache is left out here: every factor is recomputed from the
+ * neighbour's own (z, phi), which equals the reference bit for bit except in
+ * that ~1-ulp corner.
  *
  * Citations are to the reference repository (file:line).
  */
