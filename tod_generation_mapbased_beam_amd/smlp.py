@@ -37,10 +37,12 @@ gradient with respect to the channels through the transpose
 (``todg_smlp_adam``).  Every product runs on the library's BF16-MFMA GEMM;
 the host only sequences launches.
 
-Initialisation.  W3 = 0 and b3 = the factorised corrector's least-squares
-stencil (``fcorr_coef.json``): the network starts as that linear stencil and
-training moves it from there; ``init="bilinear"`` starts from plain bilinear
-interpolation (b3 = 0) instead.
+Initialisation.  W3 = 0 and b3 = the least-squares stencil over the same
+training operators (``least_squares_stencil``): the network starts as the
+best fixed linear stencil and training moves it from there;
+``init="linear"`` takes the factorised corrector's shipped coefficients
+(``fcorr_coef.json``) instead, ``init="bilinear"`` plain bilinear
+interpolation (b3 = 0).
 """
 
 from __future__ import annotations
@@ -361,9 +363,35 @@ class StencilMLPCorrector:
         final = float(np.mean([self.ratio(op, ws) for op in ops]))
         return TrainReport(len(ops), ops[0].B, step, time.time() - t0, start, final, hist)
 
+    @staticmethod
+    def least_squares_stencil(ops, ridge: float = 1e-10) -> np.ndarray:
+        """The 32 stencil coefficients (fcorr's layout: A0's weight of the
+        pixel itself fixed at 1, A1's at 0) that minimise the TOD residual
+        over the operators' samples: each free coefficient's feature is the
+        operator applied to that stencil feature, sum_p K(b, p) F_j(p); the
+        30 x 30 normal equations are solved in binary64."""
+        cols = [(0, j) for j in range(1, 16)] + [(1, j) for j in range(1, 16)]
+        G = np.zeros((len(cols), len(cols)))
+        h = np.zeros(len(cols))
+        for op in ops:
+            zero = torch.zeros(op.npix, dtype=torch.float32, device=op.F.device)
+            X = np.empty((op.B, len(cols)))
+            for k, (ch, j) in enumerate(cols):
+                fj = op.F[:, j].contiguous()
+                X[:, k] = (op.apply(fj, zero) if ch == 0 else op.apply(zero, fj)).cpu().numpy()
+            r = (op.target - op.apply(op.F[:, 0].contiguous(), zero)).cpu().numpy()
+            G += X.T @ X
+            h += X.T @ r
+        x = np.linalg.solve(G + ridge * np.trace(G) / G.shape[0] * np.eye(G.shape[0]), h)
+        coef = np.zeros(32)
+        coef[0] = 1.0
+        coef[1:16] = x[:15]
+        coef[17:32] = x[15:]
+        return coef
+
     @classmethod
     def fit(cls, nside: int = 512, n_maps: int = 8, samples_per_map: int = 40_000, epochs: int = 40,
-            lr: float = 1e-3, lr_out: float = 1e-6, init: str = "linear", seed: int = 100,
+            lr: float = 1e-3, lr_out: float = 1e-6, init: str = "lsq", seed: int = 100,
             fwhm_arcmin: float | None = None, power_cut: float = 0.99, log=None):
         """Operators of n_maps synthetic skies (seeds seed .. seed + n_maps - 1,
         spectrum of inputs.cl_spectrum, I field) at ``nside`` with
@@ -371,8 +399,8 @@ class StencilMLPCorrector:
         ``fwhm_arcmin`` (default: 30' scaled to nside 2048's pixel size) at
         nside_beam 2 nside, then ``train``.  Returns (corrector, report,
         operators)."""
-        if init not in ("linear", "bilinear"):
-            raise ValueError("init must be 'linear' or 'bilinear'")
+        if init not in ("lsq", "linear", "bilinear"):
+            raise ValueError("init must be 'lsq', 'linear' or 'bilinear'")
         dev = device()
         if fwhm_arcmin is None:
             fwhm_arcmin = 30.0 * 2048 / nside
@@ -388,7 +416,12 @@ class StencilMLPCorrector:
             ops.append(build_operator(nside, fld, dvec, dbv, params))
             if log:
                 log(f"smlp operators: sky {i + 1}/{n_maps} ({time.time() - t0:.1f}s)")
-        coef = fcorr.FactorisedCorrector.load_default().coef if init == "linear" else fcorr.identity_coef()
+        if init == "lsq":
+            coef = cls.least_squares_stencil(ops)
+        elif init == "linear":
+            coef = fcorr.FactorisedCorrector.load_default().coef
+        else:
+            coef = fcorr.identity_coef()
         corr = cls(init_params(seed, coef))
         rep = corr.train(ops, epochs, lr=lr, lr_out=lr_out, seed=seed, log=log)
         return corr, rep, ops
@@ -427,7 +460,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-out", type=float, default=1e-6)
-    ap.add_argument("--init", default="linear", choices=["linear", "bilinear"])
+    ap.add_argument("--init", default="lsq", choices=["lsq", "linear", "bilinear"])
     args = ap.parse_args(argv)
     if not args.fit:
         ap.error("nothing to do (use --fit)")
