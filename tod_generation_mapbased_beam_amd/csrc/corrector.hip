@@ -197,7 +197,8 @@ __global__ __launch_bounds__(256) void k_head(int64_t N, const float* __restrict
     if (threadIdx.x == 0 && part) part[blockIdx.x] = red[0];
 }
 
-// out[f] = sum_i D[i][f] (bias gradients), one thread per column.
+// out[f] = sum_i D[i][f] (bias gradients) for few rows: one block per
+// column, 256 threads striding over the rows, a tree reduction.
 __global__ __launch_bounds__(256) void k_colsum(int64_t rows, int F, const float* __restrict__ D,
                                                 float* __restrict__ out) {
     __shared__ float red[256];
@@ -211,6 +212,40 @@ __global__ __launch_bounds__(256) void k_colsum(int64_t rows, int F, const float
         __syncthreads();
     }
     if (threadIdx.x == 0) out[f] = red[0];
+}
+
+// Column sums for many rows (the stencil MLP's millions of map pixels),
+// F dividing 256, in two passes with a fixed order.  Pass 1:
+// block b takes the rows [b chunk, (b + 1) chunk); thread t sums column
+// t % F over every (256 / F)-th of them, so a wave's loads cover whole rows
+// (coalesced); the block's 256 / F partial sums per column are added in
+// order into part[b F + f].  Pass 2: one thread per column adds the blocks'
+// sums in order.
+__global__ __launch_bounds__(256) void k_colsum_part(int64_t rows, int F, int64_t chunk,
+                                                     const float* __restrict__ D, float* __restrict__ part) {
+    __shared__ float red[256];
+    const int G = 256 / F;
+    const int f = threadIdx.x % F, g = threadIdx.x / F;
+    const int64_t r0 = (int64_t)blockIdx.x * chunk;
+    const int64_t r1 = r0 + chunk < rows ? r0 + chunk : rows;
+    float s = 0.f;
+    for (int64_t i = r0 + g; i < r1; i += G) s += D[i * F + f];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < F) {
+        float t = 0.f;
+        for (int k = 0; k < G; ++k) t += red[k * F + threadIdx.x];
+        part[(int64_t)blockIdx.x * F + threadIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_colsum_final(int nb, int F, const float* __restrict__ part,
+                                                     float* __restrict__ out) {
+    const int f = threadIdx.x;
+    if (f >= F) return;
+    float t = 0.f;
+    for (int b = 0; b < nb; ++b) t += part[(int64_t)b * F + f];
+    out[f] = t;
 }
 
 __global__ __launch_bounds__(256) void k_sum_parts(int n, const double* __restrict__ part,
@@ -420,12 +455,35 @@ int fail(int code, const std::string& msg) { return todg_internal::set_error(cod
 
 inline unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-// K per split of the split-K products, and the largest tile count they take
-// (the 64 x 64 weight gradient).
+// K per split of the split-K products (256, or more for long K so that there
+// are at most ~256 splits: the partials then stay small and their ordered
+// reduction cheap), and the largest tile count they take (the 64 x 64
+// weight gradient).
 constexpr int kSplitKc = 256;
+constexpr int kSplitMax = 256;
 constexpr int kSplitMaxTiles = (kH / 16) * (kH / 16);
 
-inline int64_t n_splits(int64_t K) { return (K + kSplitKc - 1) / kSplitKc; }
+inline int64_t split_kc(int64_t K) {
+    const int64_t kc = (K + kSplitMax - 1) / kSplitMax;
+    return kc <= kSplitKc ? kSplitKc : (kc + 15) / 16 * 16;
+}
+inline int64_t n_splits(int64_t K) { return (K + split_kc(K) - 1) / split_kc(K); }
+
+// Column-sum workspace (floats) and the column sums themselves.
+constexpr int kColsumBlocks = 512;
+constexpr int64_t kColsumParts = (int64_t)kColsumBlocks * 64;
+
+void colsum(hipStream_t st, int64_t rows, int F, const float* D, float* out, float* part) {
+    if (rows <= 65536) {  // the per-column blocks are latency-bound beyond a few hundred rows per thread
+        hipLaunchKernelGGL(k_colsum, dim3((unsigned)F), dim3(256), 0, st, rows, F, D, out);
+        return;
+    }
+    const int64_t chunk0 = (rows + kColsumBlocks - 1) / kColsumBlocks;
+    const int64_t chunk = chunk0 < 1024 ? 1024 : chunk0;
+    const int nb = (int)((rows + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)nb), dim3(256), 0, st, rows, F, chunk, D, part);
+    hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(64), 0, st, nb, F, part, out);
+}
 
 // part: workspace for the split-K partials (n_splits(K) * tiles * 256
 // floats), or null for the one-wave-per-tile kernel.
@@ -436,7 +494,7 @@ void gemm(hipStream_t st, int M, int N, int K, const float* A, int lda, int ta, 
         const int splits = (int)n_splits(K);
         const int64_t waves = tiles * splits;
         hipLaunchKernelGGL(k_gemm_bf16_splitk, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, M, N, K,
-                           kSplitKc, splits, A, lda, ta, B, ldb, tb, part);
+                           (int)split_kc(K), splits, A, lda, ta, B, ldb, tb, part);
         hipLaunchKernelGGL(k_splitk_reduce, dim3((unsigned)(((int64_t)M * N + 255) / 256)), dim3(256), 0, st,
                            M, N, splits, part, C, ldc, alpha, beta);
         return;
@@ -480,7 +538,8 @@ extern "C" {
 // parts (doubles, within N floats), then the split-K partials of the weight
 // gradients.
 size_t todg_corr_workspace_floats(int64_t N) {
-    return (size_t)N * (4 * kH + 2 * kOut + 1) + 1024 + (size_t)n_splits(N) * kSplitMaxTiles * 256;
+    return (size_t)N * (4 * kH + 2 * kOut + 1) + 1024 + (size_t)n_splits(N) * kSplitMaxTiles * 256 +
+           (size_t)kColsumParts;
 }
 
 int todg_corr_features(int64_t nside, const float* field, int64_t N, const double* z,
@@ -525,6 +584,7 @@ int todg_corr_loss_grad(const float* params, const float* X, const float* wbil, 
     float* dO = O + N * kOut;
     double* part = reinterpret_cast<double*>(dO + N * kOut);  // 8-byte aligned: 288 N floats
     float* spl = dO + N * kOut + N + 1024;                     // split-K partials
+    float* csp = spl + n_splits(N) * kSplitMaxTiles * 256;      // column-sum partials
     forward(st, params, X, N, Z1, A1, Z2, A2, O);
     const unsigned hb = nblk(N);
     hipLaunchKernelGGL(k_head, dim3(hb), dim3(256), 0, st, N, X, wbil, that, O, grad ? dO : nullptr,
@@ -533,17 +593,17 @@ int todg_corr_loss_grad(const float* params, const float* X, const float* wbil, 
     if (grad) {
         // layer 3: dW3 = dO^T A2, db3 = colsum dO, dA2 = dO W3
         gemm(st, kOut, kH, n, dO, kOut, 1, A2, kH, 0, grad + kOffW3, kH, 1.f, 0.f, spl);
-        hipLaunchKernelGGL(k_colsum, dim3(kOut), dim3(256), 0, st, N, kOut, dO, grad + kOffB3);
+        colsum(st, N, kOut, dO, grad + kOffB3, csp);
         gemm(st, n, kH, kOut, dO, kOut, 0, params + kOffW3, kH, 0, A2, kH, 1.f, 0.f);
         hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * kH)), dim3(256), 0, st, N * kH, Z2, A2);
         // layer 2 (dZ2 in A2): dW2 = dZ2^T A1, db2, dA1 = dZ2 W2
         gemm(st, kH, kH, n, A2, kH, 1, A1, kH, 0, grad + kOffW2, kH, 1.f, 0.f, spl);
-        hipLaunchKernelGGL(k_colsum, dim3(kH), dim3(256), 0, st, N, kH, A2, grad + kOffB2);
+        colsum(st, N, kH, A2, grad + kOffB2, csp);
         gemm(st, n, kH, kH, A2, kH, 0, params + kOffW2, kH, 0, A1, kH, 1.f, 0.f);
         hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * kH)), dim3(256), 0, st, N * kH, Z1, A1);
         // layer 1 (dZ1 in A1): dW1 = dZ1^T X, db1
         gemm(st, kH, kIn, n, A1, kH, 1, X, kIn, 0, grad + kOffW1, kIn, 1.f, 0.f, spl);
-        hipLaunchKernelGGL(k_colsum, dim3(kH), dim3(256), 0, st, N, kH, A1, grad + kOffB1);
+        colsum(st, N, kH, A1, grad + kOffB1, csp);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
@@ -574,7 +634,8 @@ int todg_corr_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, co
 // Z1, A1, Z2, A2 (N x kH each), then the split-K partials of the weight
 // gradients.
 size_t todg_smlp_workspace_floats(int64_t N) {
-    return (size_t)N * 4 * todg_smlp::kH + 1024 + (size_t)n_splits(N) * kSplitMaxTiles * 256;
+    return (size_t)N * 4 * todg_smlp::kH + 1024 + (size_t)n_splits(N) * kSplitMaxTiles * 256 +
+           (size_t)kColsumParts;
 }
 
 int todg_smlp_forward(const float* params, const float* X, int64_t N, float* ws, float* O, void* stream) {
@@ -603,19 +664,20 @@ int todg_smlp_backward(const float* params, const float* X, int64_t N, float* ws
     float* Z2 = A1 + N * S::kH;
     float* A2 = Z2 + N * S::kH;
     float* spl = A2 + N * S::kH + 1024;
+    float* csp = spl + n_splits(N) * kSplitMaxTiles * 256;  // column-sum partials
     // layer 3: dW3 = dO^T A2, db3 = colsum dO, dA2 = dO W3
     gemm(st, S::kOut, S::kH, n, dO, S::kOut, 1, A2, S::kH, 0, grad + S::kOffW3, S::kH, 1.f, 0.f, spl);
-    hipLaunchKernelGGL(k_colsum, dim3(S::kOut), dim3(256), 0, st, N, S::kOut, dO, grad + S::kOffB3);
+    colsum(st, N, S::kOut, dO, grad + S::kOffB3, csp);
     gemm(st, n, S::kH, S::kOut, dO, S::kOut, 0, params + S::kOffW3, S::kH, 0, A2, S::kH, 1.f, 0.f);
     hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * S::kH)), dim3(256), 0, st, N * S::kH, Z2, A2);
     // layer 2 (dZ2 in A2): dW2 = dZ2^T A1, db2, dA1 = dZ2 W2
     gemm(st, S::kH, S::kH, n, A2, S::kH, 1, A1, S::kH, 0, grad + S::kOffW2, S::kH, 1.f, 0.f, spl);
-    hipLaunchKernelGGL(k_colsum, dim3(S::kH), dim3(256), 0, st, N, S::kH, A2, grad + S::kOffB2);
+    colsum(st, N, S::kH, A2, grad + S::kOffB2, csp);
     gemm(st, n, S::kH, S::kH, A2, S::kH, 0, params + S::kOffW2, S::kH, 0, A1, S::kH, 1.f, 0.f);
     hipLaunchKernelGGL(k_gelu_bwd, dim3(nblk(N * S::kH)), dim3(256), 0, st, N * S::kH, Z1, A1);
     // layer 1 (dZ1 in A1): dW1 = dZ1^T X, db1
     gemm(st, S::kH, S::kIn, n, A1, S::kH, 1, X, S::kIn, 0, grad + S::kOffW1, S::kIn, 1.f, 0.f, spl);
-    hipLaunchKernelGGL(k_colsum, dim3(S::kH), dim3(256), 0, st, N, S::kH, A1, grad + S::kOffB1);
+    colsum(st, N, S::kH, A1, grad + S::kOffB1, csp);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? TODG_OK : fail(TODG_ERR_HIP, hipGetErrorString(e));
 }
