@@ -137,3 +137,45 @@ def test_rotation_params_rejects_bad_arguments(case):
         a = (th, th, th, np.where(np.arange(6) == 2, np.nan, ctr))
     with pytest.raises(ValueError):
         kernels.rotation_params(*a)
+
+
+def test_smlp_channels_rejects_bad_outputs():
+    """The MLP stencil corrector's channel builder writes through raw
+    pointers: host tensors, wrong dtypes and short buffers are refused."""
+    from tod_generation_mapbased_beam_amd import smlp
+
+    nside = 4
+    npix = 12 * nside * nside
+    field = np.zeros(npix, np.float32)
+    params = np.zeros(smlp.N_PARAMS, np.float32)
+    ok = torch.zeros(npix)
+    for bad in (torch.zeros(npix), torch.zeros(npix, dtype=torch.float64), np.zeros(npix, np.float32)):
+        with pytest.raises(ValueError):
+            smlp.channels(params, nside, field, bad, ok)
+    with pytest.raises(ValueError):
+        smlp.channels(params, 3, field, ok, ok)  # nside not a power of two
+    with pytest.raises(ValueError):
+        smlp.channels(params, nside, field[:-1], ok, ok)
+    with pytest.raises(ValueError):
+        smlp.channels(params[:-1], nside, field, ok, ok)
+    with pytest.raises(ValueError):
+        smlp.channels(params, nside, field, ok, ok, stride=0)
+
+
+def test_smlp_operator_and_helpers_reject_bad_arguments(args):
+    from tod_generation_mapbased_beam_amd import smlp
+
+    vec, params, nside, mp, bv, B, S = args
+    with pytest.raises(ValueError):
+        smlp.build_operator(nside, mp[0], vec, bv, params[:5])
+    with pytest.raises(ValueError):
+        smlp.build_operator(nside, mp[0][:-1], vec, bv, params)
+    with pytest.raises(ValueError):
+        smlp.build_operator(nside, mp[0], vec[:-1], bv, params)
+    with pytest.raises(ValueError):
+        smlp.features(5, mp[0])
+    F = torch.zeros((10, 16))
+    with pytest.raises(ValueError):
+        smlp.head(F, torch.zeros((10, 32)), torch.zeros(10), torch.zeros(10))
+    with pytest.raises(ValueError):
+        smlp.head_backward(F.t(), torch.zeros(10), torch.zeros(10), torch.zeros((10, 32)))

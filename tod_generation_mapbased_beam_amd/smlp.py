@@ -97,15 +97,52 @@ def init_params(seed: int = 0, coef=None, w3_scale: float = 0.0) -> np.ndarray:
 
 
 def _check_params(params) -> torch.Tensor:
-    t = to_dev(params, torch.float32).reshape(-1).contiguous()
-    if t.numel() != N_PARAMS:
+    n = int(params.numel()) if isinstance(params, torch.Tensor) else int(np.asarray(params).size)
+    if n != N_PARAMS:
         raise ValueError(f"expected {N_PARAMS} parameters")
-    return t
+    return to_dev(params, torch.float32).reshape(-1).contiguous()
+
+
+def _check_nside(nside) -> int:
+    nside = int(nside)
+    if nside < 1 or nside > 8192 or nside & (nside - 1):
+        raise ValueError("nside must be a power of two <= 8192")
+    return nside
+
+
+def _check_dev(t, name: str, dtype=torch.float32, min_numel: int = 0, cols: int | None = None) -> None:
+    """t must be a contiguous CUDA tensor of dtype on the current device with
+    at least min_numel elements (and, with cols, a 2-D (rows, cols) shape):
+    the kernels take its raw pointer.  Checked before any device work."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor on the GPU")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must live on the GPU (got a {t.device.type} tensor)")
+    if t.device.index is not None and t.device.index != torch.cuda.current_device():
+        raise ValueError(f"{name} is on cuda:{t.device.index}, not the current device")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.numel() < min_numel:
+        raise ValueError(f"{name} holds {t.numel()} elements, needs {min_numel}")
+    if cols is not None and (t.dim() != 2 or int(t.shape[1]) != cols):
+        raise ValueError(f"{name} must be (rows, {cols})")
 
 
 def channels(params, nside: int, field, out0, out1, stride: int = 1) -> None:
     """A0 and A1 of one map field (device float32, npix) into out0[p * stride]
     and out1[p * stride] (device float32 views): one fused kernel."""
+    nside = _check_nside(nside)
+    npix = 12 * nside * nside
+    stride = int(stride)
+    if stride < 1:
+        raise ValueError("stride must be >= 1")
+    nfield = int(field.numel()) if isinstance(field, torch.Tensor) else int(np.asarray(field).size)
+    if nfield != npix:
+        raise ValueError(f"field holds {nfield} values, nside {nside} has {npix} pixels")
+    _check_dev(out0, "out0", min_numel=(npix - 1) * stride + 1)
+    _check_dev(out1, "out1", min_numel=(npix - 1) * stride + 1)
     P = _check_params(params)
     f = to_dev(field, torch.float32).reshape(-1).contiguous()
     _lib.check(_lib.lib().todg_smlp_channels(int(nside), _lib.ptr(f), _lib.ptr(P), _lib.ptr(out0),
@@ -114,6 +151,10 @@ def channels(params, nside: int, field, out0, out1, stride: int = 1) -> None:
 
 def features(nside: int, field):
     """(F, X): the per-pixel features (npix x 16) and MLP inputs (npix x 16)."""
+    nside = _check_nside(nside)
+    nfield = int(field.numel()) if isinstance(field, torch.Tensor) else int(np.asarray(field).size)
+    if nfield != 12 * nside * nside:
+        raise ValueError(f"field holds {nfield} values, nside {nside} has {12 * nside * nside} pixels")
     f = to_dev(field, torch.float32).reshape(-1).contiguous()
     npix = int(f.numel())
     F = torch.empty((npix, 16), dtype=torch.float32, device=f.device)
@@ -140,27 +181,52 @@ class Workspace:
         self.sumsq = torch.zeros(1, dtype=torch.float64, device=dev)
 
 
+def _check_rows(X, ws: Workspace) -> int:
+    _check_dev(X, "X", cols=N_IN)
+    N = int(X.shape[0])
+    if N < 1 or N > ws.N:
+        raise ValueError(f"{N} rows, the workspace holds 1 .. {ws.N}")
+    return N
+
+
 def mlp_forward(params, X, ws: Workspace):
     """O = MLP(X) over the rows of X (activations kept in ws for backward)."""
-    N = int(X.shape[0])
+    N = _check_rows(X, ws)
+    _check_dev(params, "params", min_numel=N_PARAMS)
     _lib.check(_lib.lib().todg_smlp_forward(_lib.ptr(params), _lib.ptr(X), N, _lib.ptr(ws.ws), _lib.ptr(ws.O),
                                              stream_handle()))
     return ws.O
 
 
 def mlp_backward(params, X, ws: Workspace, dO):
-    N = int(X.shape[0])
+    """Parameter gradient (ws.grad) of sum_i dO_i . o_i over the rows of the
+    preceding mlp_forward, whose activations in ws it consumes."""
+    N = _check_rows(X, ws)
+    _check_dev(params, "params", min_numel=N_PARAMS)
+    _check_dev(dO, "dO", min_numel=N * N_OUT)
     _lib.check(_lib.lib().todg_smlp_backward(_lib.ptr(params), _lib.ptr(X), N, _lib.ptr(ws.ws), _lib.ptr(dO),
                                               _lib.ptr(ws.grad), stream_handle()))
     return ws.grad
 
 
 def head(F, O, A0, A1):
+    """Channels A0, A1 from the MLP outputs O and the features F."""
+    _check_dev(F, "F", cols=16)
+    N = int(F.shape[0])
+    _check_dev(O, "O", min_numel=N * N_OUT)
+    _check_dev(A0, "A0", min_numel=N)
+    _check_dev(A1, "A1", min_numel=N)
     _lib.check(_lib.lib().todg_smlp_head(int(F.shape[0]), _lib.ptr(F), _lib.ptr(O), _lib.ptr(A0), _lib.ptr(A1),
                                           stream_handle()))
 
 
 def head_backward(F, G0, G1, dO):
+    """dO from the loss gradients G0, G1 with respect to the channels."""
+    _check_dev(F, "F", cols=16)
+    N = int(F.shape[0])
+    _check_dev(G0, "G0", min_numel=N)
+    _check_dev(G1, "G1", min_numel=N)
+    _check_dev(dO, "dO", min_numel=N * N_OUT)
     _lib.check(_lib.lib().todg_smlp_head_backward(int(F.shape[0]), _lib.ptr(F), _lib.ptr(G0), _lib.ptr(G1),
                                                    _lib.ptr(dO), stream_handle()))
 
@@ -193,6 +259,10 @@ class TodOperator:
 
     def apply(self, A0, A1, out=None):
         """y = K0 A0 + K1 A1 (float64, B)."""
+        _check_dev(A0, "A0", min_numel=self.npix)
+        _check_dev(A1, "A1", min_numel=self.npix)
+        if out is not None:
+            _check_dev(out, "out", dtype=torch.float64, min_numel=self.B)
         y = torch.empty(self.B, dtype=torch.float64, device=self.v0.device) if out is None else out
         _lib.check(_lib.lib().todg_csr_rows2(self.B, _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.v0),
                                               _lib.ptr(self.v1), _lib.ptr(A0), _lib.ptr(A1), None, 1.0,
@@ -202,6 +272,11 @@ class TodOperator:
     def residual(self, A0, A1, scale, resid, part, sumsq):
         """resid = scale (K0 A0 + K1 A1 - target) (float32), sumsq = its
         unscaled sum of squares."""
+        _check_dev(A0, "A0", min_numel=self.npix)
+        _check_dev(A1, "A1", min_numel=self.npix)
+        _check_dev(resid, "resid", min_numel=self.B)
+        _check_dev(part, "part", dtype=torch.float64, min_numel=int(_lib.lib().todg_csr_rows2_parts(self.B)))
+        _check_dev(sumsq, "sumsq", dtype=torch.float64, min_numel=1)
         _lib.check(_lib.lib().todg_csr_rows2(self.B, _lib.ptr(self.rowptr), _lib.ptr(self.col), _lib.ptr(self.v0),
                                               _lib.ptr(self.v1), _lib.ptr(A0), _lib.ptr(A1), _lib.ptr(self.target),
                                               float(scale), None, _lib.ptr(resid), _lib.ptr(part), _lib.ptr(sumsq),
@@ -209,6 +284,9 @@ class TodOperator:
 
     def adjoint(self, r, G0, G1):
         """G0 = K0^T r, G1 = K1^T r (float32, npix)."""
+        _check_dev(r, "r", min_numel=self.B)
+        _check_dev(G0, "G0", min_numel=self.npix)
+        _check_dev(G1, "G1", min_numel=self.npix)
         _lib.check(_lib.lib().todg_csr_adjoint2(self.npix, _lib.ptr(self.rowptr_t), _lib.ptr(self.col_t),
                                                  _lib.ptr(self.v0_t), _lib.ptr(self.v1_t), _lib.ptr(r), _lib.ptr(G0),
                                                  _lib.ptr(G1), stream_handle()))
@@ -220,15 +298,28 @@ def build_operator(nside: int, field, vec_orig, beam_vals, params, chunk: int = 
     (beam value x bilinear weight, and the same times (1 - g) g, g the lower
     ring's share of the four) summed per map pixel.  Deterministic: the
     per-pixel sums are differences of a binary64 prefix sum in sorted order."""
+    nside = _check_nside(nside)
+    nfield = int(field.numel()) if isinstance(field, torch.Tensor) else int(np.asarray(field).size)
+    if nfield != 12 * nside * nside:
+        raise ValueError("field size does not match nside")
+    if len(params) != 6:
+        raise ValueError("params must be the six rotation operand arrays")
+    sizes = [int(p.numel()) if isinstance(p, torch.Tensor) else int(np.asarray(p).size) for p in params]
+    B = sizes[1]
+    if sizes != [3 * B, B, B, 3 * B, B, B] or B < 1:
+        raise ValueError("rotation operands: axes (B, 3), cos_a, sin_a (B), ax_pts (B, 3), cos_p, sin_p (B)")
+    nvec = int(vec_orig.numel()) if isinstance(vec_orig, torch.Tensor) else int(np.asarray(vec_orig).size)
+    nbv = int(beam_vals.numel()) if isinstance(beam_vals, torch.Tensor) else int(np.asarray(beam_vals).size)
+    if nbv < 1 or nvec != 3 * nbv:
+        raise ValueError("vec_orig must be (S, 3) for beam_vals (S,)")
+    if int(chunk) < 1:
+        raise ValueError("chunk must be >= 1")
     dev = device()
     f = to_dev(field, torch.float32).reshape(-1).contiguous()
     npix = int(f.numel())
-    if npix != 12 * nside * nside:
-        raise ValueError("field size does not match nside")
     vec = to_dev(vec_orig, torch.float32).contiguous()
     bv = to_dev(beam_vals, torch.float32).contiguous()
     P = [to_dev(p, torch.float32).contiguous() for p in params]
-    B = int(P[1].numel())
     S = int(bv.numel())
     counts, cols, vals0, vals1 = [], [], [], []
     for b0 in range(0, B, chunk):
