@@ -47,6 +47,10 @@ VALU_PEAK_GINST = 1024 * 2.4 / 4.0
 # the factorised gather; learned-mlp: the MLP corrector's 4 x 4 neighbourhood)
 BYTES_PER_BPS = {"bilinear": 48, "nearest": 12, "learned": 96, "learned-linear": 96, "learned-mlp": 192}
 DTYPE = "mixed: f64 rotation, index path and accumulation; f32 map, weights and near-boresight spin-2"
+# contiguous prefix of the day on which the autotuner times the variants (a
+# short prefix over-weights the launch's ramp and tail: on 131072 samples the
+# ranking of the tile variants did not hold for the whole day)
+TUNE_SAMPLES = 524288
 
 
 def log(msg):
@@ -243,8 +247,8 @@ def main():
             from tod_generation_mapbased_beam_amd import autotune
 
             variant, tms = autotune.tune_single(dvec, dbv, dparams, dmap, mode=args.mode,
-                                                max_samples=131072, reps=3)
-            variant_note = "autotuned on the first 131072 samples: " + ", ".join(
+                                                max_samples=TUNE_SAMPLES, reps=3)
+            variant_note = f"autotuned on the first {TUNE_SAMPLES} samples: " + ", ".join(
                 f"{k}: {v:.2f} ms" for k, v in sorted(tms.items()))
             log(f"autotune: variant {variant} ({variant_note})")
     prep_ms = None
@@ -363,7 +367,7 @@ def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fma
 
     from tod_generation_mapbased_beam_amd import autotune, fcorr, inputs, kernels, smlp
 
-    pv, _ = autotune.tune_single(dvec, dbv, dparams, dmap, mode="bilinear", max_samples=131072, reps=3)
+    pv, _ = autotune.tune_single(dvec, dbv, dparams, dmap, mode="bilinear", max_samples=TUNE_SAMPLES, reps=3)
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
     ts = []
     for i in range(4):
@@ -433,11 +437,11 @@ def bench_day(args, dev):
         driver.write_scan_day(folder, 0, th, ph, ps)
         bd = {"IQU": driver.beam_entry(vec, bv, [0, 1, 2], mp)}
         # the autotuner's choice for this shape, as beam_tod_batch reads it
-        params = [to_dev(p, torch.float32) for p in R.rotation_params_for_pointing(th[:131072], ph[:131072],
-                                                                                    ps[:131072])]
+        params = [to_dev(p, torch.float32) for p in R.rotation_params_for_pointing(
+            th[:TUNE_SAMPLES], ph[:TUNE_SAMPLES], ps[:TUNE_SAMPLES])]
         tv, _ = autotune.tune_single(to_dev(vec, torch.float32), to_dev(bv, torch.float32), params,
                                      DeviceMap(bd["IQU"]["mp_stacked"], 1, 2), mode="bilinear",
-                                     max_samples=131072, reps=3)
+                                     max_samples=TUNE_SAMPLES, reps=3)
         log(f"day: S={S} B={B}, tuned variant {tv}, prep {args.prep}")
 
         def step():
