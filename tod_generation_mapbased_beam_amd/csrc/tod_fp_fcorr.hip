@@ -18,16 +18,25 @@ hipError_t launch_m4(const TodArgs& a, bool qu, int CS, hipStream_t st) {
 }
 
 // Value of ring r2 at longitude phi by linear interpolation between the two
-// ring pixels whose centres bracket phi.
+// ring pixels whose centres bracket phi.  The callers' longitudes lie within
+// a few slots of [0, 2 pi) (a pixel's own longitude plus at most 3 of its
+// ring's slots, which are at most 1.75 slots of a ring 3 rings away), so one
+// conditional wrap replaces the 64-bit modulo; anything else takes the
+// general remainder.
 __device__ __forceinline__ double ring_lerp(const RingEntry* __restrict__ rt, const float* __restrict__ m,
                                             int r2, double phi) {
     const RingEntry& e = rt[r2];
     const double t = (phi - e.phi0) * e.inv_dphi;
     const double jf = floor(t);
     const double f = t - jf;
-    int64_t j = (int64_t)jf % e.n;
-    if (j < 0) j += e.n;
-    const int64_t j1 = j + 1 == e.n ? 0 : j + 1;
+    const int n = e.n;
+    int j = (int)jf;
+    if (j < 0)
+        j += n;
+    else if (j >= n)
+        j -= n;
+    if ((unsigned)j >= (unsigned)n) j = (int)(((int64_t)jf % n + n) % n);
+    const int j1 = j + 1 == n ? 0 : j + 1;
     return (1.0 - f) * (double)m[e.first + j] + f * (double)m[e.first + j1];
 }
 
