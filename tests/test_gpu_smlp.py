@@ -251,3 +251,43 @@ def test_shipped_parameters(torch):
     linear = smlp.StencilMLPCorrector(smlp.init_params(0, fcorr.FactorisedCorrector.load_default().coef)).ratio(op, ws)
     print(f"held-out ratio: shipped network {shipped:.4f}, least-squares stencil {linear:.4f}")
     assert shipped <= 0.2 and shipped <= linear * 1.01
+
+
+@pytest.mark.parametrize("nside", [1, 2, 4])
+def test_tiny_maps_match_the_oracle(torch, nside):
+    """Maps smaller than one wave's 64 pixels, and rings within 3 of a pole
+    everywhere (features with ring offsets are 0 there)."""
+    from tod_generation_mapbased_beam_amd import smlp
+
+    m = np.random.default_rng(nside).standard_normal(12 * nside * nside).astype(np.float32)
+    flat = smlp.init_params(5, np.random.default_rng(1).normal(0.0, 0.3, 32), 1.0)
+    A0 = torch.empty(m.size, device="cuda")
+    A1 = torch.empty(m.size, device="cuda")
+    smlp.channels(flat, nside, torch.as_tensor(m, device="cuda"), A0, A1, 1)
+    F, z = SO.features(nside, m)
+    o, _ = SO.forward(flat, SO.inputs(F, z), bf16=True)
+    R0, R1 = SO.head(F, o)
+    assert np.max(np.abs(A0.cpu().numpy() - R0)) <= 1e-4 * max(1.0, np.max(np.abs(R0)))
+    assert np.max(np.abs(A1.cpu().numpy() - R1)) <= 1e-4 * max(1.0, np.max(np.abs(R1)))
+
+
+def test_packed_maps_hold_each_fields_channels(torch):
+    """The I/Q/U sky's packed channel maps (slot stride 4, Q/U in slots 0/1)
+    hold, slot by slot, the channels built field by field."""
+    from tod_generation_mapbased_beam_amd import inputs, smlp
+
+    nside = 32
+    sky = torch.as_tensor(inputs.synfast_iqu(nside, seed=9), dtype=torch.float32, device="cuda")
+    net = smlp.StencilMLPCorrector.load_default()
+    maps = net.maps(sky, 1, 2)
+    assert maps.cs == 4
+    p0 = maps.packed0.view(-1, 4)
+    p1 = maps.packed1.view(-1, 4)
+    for k, c in enumerate(maps.perm):
+        if c < 0:
+            assert float(torch.max(torch.abs(p0[:, k]))) == 0.0
+            continue
+        a0 = torch.empty(sky.shape[1], device="cuda")
+        a1 = torch.empty_like(a0)
+        net.channels(nside, sky[c].contiguous(), a0, a1)
+        assert torch.equal(p0[:, k], a0) and torch.equal(p1[:, k], a1)
