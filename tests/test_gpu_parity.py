@@ -241,6 +241,39 @@ def test_accumulates_in_place(K):
     np.testing.assert_allclose(tod, 2 * first, rtol=0, atol=1e-10)
 
 
+@pytest.mark.parametrize("variant", [0, 8, 2])
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
+def test_empty_and_single_inputs(K, mode, variant):
+    """No samples or no beam pixels leave tod as it is (the reference's
+    prange over an empty range, tod_bilinear.py:233); one sample, one beam
+    pixel, or both, give the oracle's TOD.  With one sample a component's
+    RMS over samples is that one value, which the spin-2 mixing of Q and U
+    can make arbitrarily small, so those cases are normalised by the RMS of
+    the map field instead (the scale of a normalised beam's TOD)."""
+    c = small_config()
+    nside, mp, vec, bv, S = c["nside"], c["map"], c["vec"], c["bv"], c["S"]
+    fn = K._gather_accum_fused_jit if mode == "bilinear" else K._gather_accum_nearest_jit
+    p0 = tuple(p[:0] for p in c["params"])
+    tod = np.zeros((3, 0))
+    fn(vec, *p0, nside, mp, bv, 0, S, tod, 1, 2, variant=variant)
+    B = 64
+    pB = tuple(p[:B] for p in c["params"])
+    tod = np.full((3, B), 0.25)
+    fn(vec[:0], *pB, nside, mp, bv[:0], B, 0, tod, 1, 2, variant=variant)
+    assert np.array_equal(tod, np.full((3, B), 0.25))
+    for b, s in ((1, S), (B, 1), (1, 1)):
+        args = (vec[:s], tuple(p[:b] for p in c["params"]), nside, mp, bv[:s], b, s, 1, 2)
+        g = _gpu_tod(K, mode, *args, variant=variant)
+        r = _cpu_tod(mode, *args)
+        if b == 1:
+            scale = np.sqrt(np.mean(mp.astype(np.float64) ** 2, axis=1, keepdims=True))
+            err = float(np.max(np.abs(g - r) / scale))
+        else:
+            err = rel_err(g, r)
+        print(f"{mode} variant {variant} B={b} S={s}: rel err {err:.3e}")
+        assert err <= TOL
+
+
 def test_zero_beam_and_constant_map(K):
     rng = np.random.default_rng(41)
     nside, B, S = 32, 16, 40
