@@ -194,6 +194,9 @@ def main():
     ap.add_argument("--judge-samples", type=int, default=100_000,
                     help="learned: samples of the day (evenly spaced) on which the residual against the "
                          "bicubic judge is measured on a synthetic nside-2048 sky; 0 skips it")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="bilinear: skip the learned corrector's time and residual that the line carries "
+                         "beside the plain kernel's throughput (the metric's second half)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calib", action="store_true",
@@ -317,6 +320,9 @@ def main():
     if args.mode in ("learned", "learned-linear"):
         learned = learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms,
                                  prep_ms, B, S)
+    elif args.mode == "bilinear" and not args.no_learned:
+        learned = learned_beside_plain(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, mp, kern_ms,
+                                       variant, B, S)
     cpu = None
     if not args.no_cpu and args.mode in ("bilinear", "nearest"):
         mp_host = mp.cpu().numpy()
@@ -357,28 +363,68 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms, prep_ms, B, S):
-    """The corrector's line (the MLP stencil corrector, or the least-squares
-    stencil's channels): the plain bilinear kernel timed on the same workload
-    (the autotuner's variant), the time ratio with and without the one-off
-    channel build, and the RMS / max residual against the bicubic judge on a
-    synthetic sky, relative to bilinear's."""
+def learned_beside_plain(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, mp, plain_ms, plain_variant,
+                         B, S):
+    """The metric's second half on the plain bilinear line: the MLP stencil
+    corrector (the package's trained parameters) on the same day and map --
+    its channel build, its TOD kernel timed with HIP events over three
+    launches after an untimed one, the ratio to the plain kernel's time
+    measured above -- and the residual against the bicubic judge."""
     import torch
 
-    from tod_generation_mapbased_beam_amd import autotune, fcorr, inputs, kernels, smlp
+    from tod_generation_mapbased_beam_amd import fcorr, smlp
 
-    pv, _ = autotune.tune_single(dvec, dbv, dparams, dmap, mode="bilinear", max_samples=TUNE_SAMPLES, reps=3)
+    fc = smlp.StencilMLPCorrector.load_default()
+    fc.maps(mp, 1, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fmaps = fc.maps(mp, 1, 2)
+    torch.cuda.synchronize()
+    prep_ms = (time.perf_counter() - t0) * 1e3
     tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
     ts = []
     for i in range(4):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        kernels._gather_accum_fused_jit(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, variant=pv)
+        fcorr.gather_accum_fcorr(dvec, *dparams, args.nside, fmaps, dbv, B, S, tod)
         e1.record()
         torch.cuda.synchronize()
         if i:
             ts.append(e0.elapsed_time(e1))
-    plain_ms = float(np.mean(ts))
+    del tod
+    learned_args = argparse.Namespace(**{**vars(args), "mode": "learned"})
+    return learned_extras(learned_args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps,
+                          float(np.mean(ts)), prep_ms, B, S, plain=(plain_ms, plain_variant))
+
+
+def learned_extras(args, dev, vec, bv, params, dvec, dbv, dparams, dmap, fc, fmaps, kern_ms, prep_ms, B, S,
+                   plain=None):
+    """The corrector's line (the MLP stencil corrector, or the least-squares
+    stencil's channels): the plain bilinear kernel timed on the same workload
+    (the autotuner's variant; or `plain` = (its kernel ms, variant) when the
+    caller has timed it), the time ratio with and without the one-off channel
+    build, and the RMS / max residual against the bicubic judge on a
+    synthetic sky, relative to bilinear's."""
+    import torch
+
+    from tod_generation_mapbased_beam_amd import autotune, fcorr, inputs, kernels, smlp
+
+    if plain is None:
+        pv, _ = autotune.tune_single(dvec, dbv, dparams, dmap, mode="bilinear", max_samples=TUNE_SAMPLES,
+                                     reps=3)
+        tod = torch.zeros((3, B), dtype=torch.float64, device=dev)
+        ts = []
+        for i in range(4):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            kernels._gather_accum_fused_jit(dvec, *dparams, args.nside, dmap, dbv, B, S, tod, 1, 2, variant=pv)
+            e1.record()
+            torch.cuda.synchronize()
+            if i:
+                ts.append(e0.elapsed_time(e1))
+        plain_ms = float(np.mean(ts))
+    else:
+        plain_ms, pv = plain
     if args.mode == "learned":
         method = ("MLP stencil corrector: 16-64-64-32 GELU network per map pixel on BF16 MFMA "
                   "(todg_smlp_channels) + todg_gather_accum_fcorr")

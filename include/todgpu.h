@@ -219,6 +219,22 @@ int todg_beam_bell(const double *x, const double *w, int64_t N, int lmax, double
 int todg_beam_bell_noalloc(const double *x, const double *w, int64_t N, int lmax, double *bell,
                            void *stream);
 
+/* Ring sums of a spin-0 spherical-harmonic synthesis,
+ *   F_m(theta_r) = sum_{l=m}^{lmax} a_lm lambda_lm(theta_r),
+ * for K <= 4 fields at once (lambda_lm orthonormal associated Legendre
+ * functions, three-term recurrence in l with a rescaled mantissa so polar
+ * rings at high m neither underflow nor overflow: the mantissa is divided by
+ * `big` when it exceeds it and the exponent grows by logbig = log(big)).  The
+ * step of healpy's alm2map / synfast before the per-ring FFT; the reference
+ * reads its input skies from files and takes such synthetic ones in its
+ * configurations and tests, so this has no counterpart in its own code.
+ * Device pointers: alm K x (lmax+1) x (lmax+1) complex binary64 as (re, im)
+ * pairs, laid out [k][m][l] (entries with l < m unused); x = cos theta_r and
+ * st = sin theta_r, R each; logmm: lmax + 1 values log sqrt((2m+1)/(4 pi)
+ * prod_{j<=m} (2j-1)/(2j)); F: K x (lmax+1) x R complex binary64, [k][m][r]. */
+int todg_ring_sums(const double *alm, int K, int lmax, const double *x, const double *st, int64_t R,
+                   const double *logmm, double big, double logbig, double *F, void *stream);
+
 /* ------------------------------------------------------------------------
  * Learned interpolation corrector (no counterpart in the reference: the
  * build's corrector of bilinear interpolation toward the bicubic judge).

@@ -51,6 +51,7 @@ SIGNATURES = {
     "todg_beam_bell_workspace": (ctypes.c_size_t, [_i64, _i32]),
     "todg_beam_bell": (_i32, [_p, _p, _i64, _i32, _p, _p, _p]),
     "todg_beam_bell_noalloc": (_i32, [_p, _p, _i64, _i32, _p, _p]),
+    "todg_ring_sums": (_i32, [_p, _i32, _i32, _p, _p, _i64, _p, ctypes.c_double, ctypes.c_double, _p, _p]),
     "todg_corr_features": (_i32, [_i64, _p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "todg_corr_workspace_floats": (ctypes.c_size_t, [_i64]),
     "todg_corr_forward": (_i32, [_p, _p, _i64, _p, _p, _p]),

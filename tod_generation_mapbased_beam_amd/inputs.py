@@ -46,7 +46,20 @@ def _ring_sums(alm_list, lmax, thetas, dev, spin_fields):
     and m, for each a_lm set in alm_list ((lmax+1, lmax+1) complex, [l, m]).
     Normalised associated-Legendre recurrence in l at fixed m, vectorised over
     (ring, m), with a per-element exponent so polar rings at high m neither
-    underflow nor lose precision."""
+    underflow nor lose precision.  On a GPU device the sums run in the
+    library's kernel (todg_ring_sums), which performs the same binary64
+    operations in the same order."""
+    dev = torch.device(dev)
+    if dev.type == "cuda":
+        return _ring_sums_gpu(alm_list, lmax, thetas, dev)
+    return _ring_sums_torch(alm_list, lmax, thetas, dev)
+
+
+def _ring_sums_torch(alm_list, lmax, thetas, dev):
+    """_ring_sums as torch tensor operations on ``dev``, vectorised over
+    (ring, m): the host path, and the reference the GPU kernel is tested
+    against on the same device."""
+    dev = torch.device(dev)
     R = thetas.shape[0]
     M = lmax + 1
     x = torch.as_tensor(np.cos(thetas), dtype=torch.float64, device=dev)[:, None]  # (R,1)
@@ -61,7 +74,7 @@ def _ring_sums(alm_list, lmax, thetas, dev, spin_fields):
     outs = [torch.zeros((R, M), dtype=torch.complex128, device=dev) for _ in alm_list]
     alms = [torch.as_tensor(a, dtype=torch.complex128, device=dev) for a in alm_list]
     mi = torch.arange(M, device=dev)
-    big = 1e150
+    big = _BIG
     logbig = math.log(big)
     for k in range(0, M):
         # l = m + k for every m; valid where l <= lmax
@@ -106,6 +119,38 @@ def _alm_realisation(cl, lmax, rng):
     return a
 
 
+_BIG = 1e150
+
+
+def _ring_sums_gpu(alm_list, lmax, thetas, dev):
+    """_ring_sums on the GPU: one launch of todg_ring_sums for up to four
+    a_lm sets at a time; returns (R, M) complex128 device tensors."""
+    from . import _lib
+    from .device import stream_handle
+
+    R = int(thetas.shape[0])
+    M = lmax + 1
+    x = torch.as_tensor(np.cos(thetas), dtype=torch.float64, device=dev).contiguous()
+    st = torch.as_tensor(np.sin(thetas), dtype=torch.float64, device=dev).contiguous()
+    logmm = torch.as_tensor(_lambda_mm_log(np.arange(M)), dtype=torch.float64, device=dev).contiguous()
+    outs = []
+    for g0 in range(0, len(alm_list), 4):
+        group = alm_list[g0:g0 + 4]
+        # [k][m][l]: a lane's reads along l are contiguous
+        alm = torch.stack([torch.as_tensor(np.asarray(a), dtype=torch.complex128, device=dev).transpose(0, 1)
+                           for a in group]).contiguous()
+        if tuple(alm.shape[1:]) != (M, M):
+            raise ValueError(f"a_lm sets must be ({M}, {M})")
+        F = torch.empty((len(group), M, R), dtype=torch.complex128, device=dev)
+        with torch.cuda.device(dev):
+            _lib.check(_lib.lib().todg_ring_sums(
+                _lib.ptr(alm), len(group), lmax, _lib.ptr(x), _lib.ptr(st), R, _lib.ptr(logmm), _BIG,
+                math.log(_BIG), _lib.ptr(F), stream_handle()))
+        del alm
+        outs += [F[k].transpose(0, 1) for k in range(len(group))]
+    return outs
+
+
 def synthesize_map(nside: int, alm_list, lmax: int, device=None):
     """Spin-0 synthesis of one or more a_lm sets on the RING grid -> (K, npix) float64."""
     dev = torch.device(device) if device is not None else torch.device("cpu")
@@ -113,7 +158,7 @@ def synthesize_map(nside: int, alm_list, lmax: int, device=None):
     thetas = np.arccos(z)
     F = _ring_sums(alm_list, lmax, thetas, dev, None)
     out = np.zeros((len(alm_list), hpx.npix(nside)), dtype=np.float64)
-    Fh = [f.cpu().numpy() for f in F]
+    Fh = [f.contiguous().cpu().numpy() for f in F]
     mm = np.arange(lmax + 1)
     for r in range(len(n)):
         nr = int(n[r])
