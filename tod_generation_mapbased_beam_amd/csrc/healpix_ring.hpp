@@ -478,6 +478,23 @@ __device__ __forceinline__ float4 load_px(const float* __restrict__ map, int64_t
     }
 }
 
+// One pixel of the packed map by a 32-bit index (npix < 2^31), clamped to
+// pmax = npix - 1 by one unsigned min: for indices that are in range by
+// construction (the footprint sweep's laid-out cells), where the 64-bit
+// clamp above would cost its compare-and-select chain per pixel.
+template <int CS>
+__device__ __forceinline__ float4 load_px_u32(const float* __restrict__ map, uint32_t p, uint32_t pmax) {
+    p = min(p, pmax);
+    if constexpr (CS == 4) {
+        return *(reinterpret_cast<const float4*>(map) + (uint64_t)p);
+    } else if constexpr (CS == 2) {
+        float2 v = *(reinterpret_cast<const float2*>(map) + (uint64_t)p);
+        return make_float4(v.x, v.y, 0.f, 0.f);
+    } else {
+        return make_float4(map[(uint64_t)p], 0.f, 0.f, 0.f);
+    }
+}
+
 // ------------------------------------------------------------ bicubic judge
 
 // Keys cubic convolution kernel, a = -0.5 (Catmull-Rom), support |t| < 2.

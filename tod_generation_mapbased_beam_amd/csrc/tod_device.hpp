@@ -2087,6 +2087,7 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
     const int wave = threadIdx.x >> 6;
     const int total = __builtin_amdgcn_readfirstlane(fr->total);
     const int coinc = __builtin_amdgcn_readfirstlane(fr->coinc_cell);
+    const uint32_t pmax = (uint32_t)(a.npix - 1);
     const FpSpin sp = fp_spin_of(fr);
     for (int base = wave * 64; base < total; base += kFpThreads * kP2) {
         double wsum[kP2], w1sum[kP2];
@@ -2116,8 +2117,12 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
                 const int n = R.n(r);
                 if (pos >= n) pos -= n;
                 kk[u] = k;
-                mv[u] = load_px<CS>(a.map, (int64_t)R.first(r) + pos, a.npix);
-                if constexpr (FC) mv1[u] = load_px<CS>(a.map1, (int64_t)R.first(r) + pos, a.npix);
+                // cell j < total lies in ring r's window: 0 <= k < width <= n
+                // and 0 <= i0 < n, so pos is in [0, n) and the pixel in the
+                // ring (npix < 2^31); the clamp only backs that up
+                const uint32_t pix = (uint32_t)(R.first(r) + pos);
+                mv[u] = load_px_u32<CS>(a.map, pix, pmax);
+                if constexpr (FC) mv1[u] = load_px_u32<CS>(a.map1, pix, pmax);
             }
         }
 #pragma unroll
