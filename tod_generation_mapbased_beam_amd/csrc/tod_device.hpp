@@ -102,6 +102,18 @@ __device__ __forceinline__ double grid_magic(double sum_abs_beam) {
 
 __device__ __forceinline__ double qround(double x, double M) { return (x + M) - M; }
 
+// A weight-grid cell holds its sum as an integer number of quanta q (two's
+// complement): x rounded to the grid is bits(x + M) - bits(M) quanta, and a
+// cell of n quanta is worth bits^-1(bits(M) + n) - M = n q exactly, since
+// |n q| stays below 2^E / 64 (at most the beam's sum of |values|).  Integer
+// sums are exact and order-free like the binary64 sums of grid multiples,
+// with the same values; LDS adds them with ds_add_u64, far cheaper than
+// ds_add_f64.  A beam with M = 0 (no grid) gets no layout (fp_layout).
+typedef unsigned long long WCell;
+__device__ __forceinline__ WCell wbits(double x) { return __builtin_bit_cast(WCell, x); }
+__device__ __forceinline__ WCell wq(double x, double M) { return wbits(x + M) - wbits(M); }
+__device__ __forceinline__ double wval(WCell n, double M) { return __builtin_bit_cast(double, wbits(M) + n) - M; }
+
 struct Sample {
     double kx, ky, kz, ca, sa, oma, bx, by, bz, cp, sp, omp;
     double z_pts, sth_pts, phi_pts;
@@ -366,7 +378,6 @@ constexpr int kOffItem = kOffJob + 2 * 64;
 constexpr int kItemBytes = 256;
 constexpr int kOffW = kOffItem + kItemBytes;
 // weight-grid cell type (beam value x interpolation weight sums)
-typedef double WCell;
 // ring of each weight-grid cell (footprint-relative, < kFpMaxRings <= 256),
 // one table per buffer
 constexpr int kOffRtab = kOffW + kFpCap * (int)sizeof(WCell);
@@ -715,7 +726,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
         const int lr = ring - ir_lo;
         const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
         if (li >= 0) {
-            atomicAdd(&W[li], (WCell)w);
+            atomicAdd(&W[li], wq(w, M));
         } else {
             const Acc4 d = fr->direct ? fp_direct<QU, CS>(a, sp, ring, pos, w)
                                       : fp_outside<QU, CS>(a, sp, ring, pos, w);
@@ -735,8 +746,8 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
             const int lr = o.ring[k] - ir_lo;
             const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
             if (li >= 0) {
-                atomicAdd(&W[li], (WCell)w);
-                atomicAdd(&W1[li], (WCell)w1);
+                atomicAdd(&W[li], wq(w, M));
+                atomicAdd(&W1[li], wq(w1, M));
             } else {
                 const Acc4 d = fr->direct ? fp_direct_fc<QU, CS>(a, sp, o.ring[k], o.pos[k], w, w1)
                                           : fp_outside<QU, CS, true>(a, sp, o.ring[k], o.pos[k], w, w1);
@@ -967,7 +978,9 @@ __device__ __forceinline__ bool fp_layout(const TodArgs& a, int64_t b, const Bea
     ir_lo = ir_lo < 1 ? 1 : ir_lo;
     ir_hi = ir_hi > 4 * nside - 1 ? 4 * nside - 1 : ir_hi;
     int nr = ir_hi - ir_lo + 1;
-    if (nr > kFpMaxRings || nr < 0) nr = 0;  // no layout: everything goes direct
+    // no layout (everything goes direct): too many rings, or a beam without a
+    // weight grid (sum of |values| zero or not finite: M = 0)
+    if (nr > kFpMaxRings || nr < 0 || !(bf.M > 0.0)) nr = 0;
 
     // Composite rotation: column j of R2 R1 is the reference rotation of the
     // unit vector e_j; its rows are then turned by -lam about z, so that the
@@ -1265,20 +1278,23 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     const double ua = wad + M, va = fma(wad, fa, M);
     const double ub = wbd + M, vb = fma(wbd, fb, M);
     const double d0 = ua - va, d1 = va - M, d2 = ub - vb, d3 = vb - M;
+    // the same four values as integer quanta: differences of the bit
+    // patterns of ua, va, ub, vb and M, which share one binade
     WCell* ca = W + ia;
     WCell* cb = W + ib;
-    atomicAdd(ca, (WCell)d0);
-    atomicAdd(ca + 1, (WCell)d1);
-    atomicAdd(cb, (WCell)d2);
-    atomicAdd(cb + 1, (WCell)d3);
+    const WCell bM = wbits(M), bva = wbits(va), bvb = wbits(vb);
+    atomicAdd(ca, wbits(ua) - bva);
+    atomicAdd(ca + 1, bva - bM);
+    atomicAdd(cb, wbits(ub) - bvb);
+    atomicAdd(cb + 1, bvb - bM);
     if constexpr (FC) {
         // the correction grid: each deposit times (1 - g) g, grid-rounded
         const double uc = (double)fmaf(-gfr, gfr, gfr);
         WCell* W1 = fp_W1();
-        atomicAdd(W1 + ia, (WCell)qround(d0 * uc, M));
-        atomicAdd(W1 + ia + 1, (WCell)qround(d1 * uc, M));
-        atomicAdd(W1 + ib, (WCell)qround(d2 * uc, M));
-        atomicAdd(W1 + ib + 1, (WCell)qround(d3 * uc, M));
+        atomicAdd(W1 + ia, wq(d0 * uc, M));
+        atomicAdd(W1 + ia + 1, wq(d1 * uc, M));
+        atomicAdd(W1 + ib, wq(d2 * uc, M));
+        atomicAdd(W1 + ib + 1, wq(d3 * uc, M));
     }
     if constexpr (DUMP) {
         dd.pix[4 * di] = fp_cell_pixel(R, la, ia);
@@ -1388,7 +1404,7 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
     ok &= (best - other > kCertNearest) & ((w1 ? lead[1] : lead[0]) > kCertNearest);
     if (!ok) return false;
     const double w = qround(bv, M);
-    atomicAdd(W + cell, (WCell)w);
+    atomicAdd(W + cell, wq(bv, M));
     if constexpr (DUMP) {
         dd.pix[4 * di] = fp_cell_pixel(R, la + (w1 ? 1 : 0), cell);
         dd.w[4 * di] = w;
@@ -1891,7 +1907,7 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
             const int lr = ring[r] - ir_lo;
             const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
             if (li >= 0) {
-                atomicAdd(&W[li], (WCell)w);
+                atomicAdd(&W[li], wq(w, M));
             } else {
                 const Acc4 d = fr->direct ? fp_direct<QU, CS>(a, sp, ring[r], pos, w)
                                           : fp_outside<QU, CS>(a, sp, ring[r], pos, w);
@@ -2045,12 +2061,12 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
                 for (int r = 0; r < 4; ++r)
 #pragma unroll
                     for (int t = 0; t < 4; ++t)
-                        atomicAdd(W + G.cell[r] + t, (WCell)qround(w16[4 * r + t], M));
+                        atomicAdd(W + G.cell[r] + t, wq(w16[4 * r + t], M));
             } else {
-                atomicAdd(W + G.cell[1] + 1, (WCell)qround(w16[5], M));
-                atomicAdd(W + G.cell[1] + 2, (WCell)qround(w16[6], M));
-                atomicAdd(W + G.cell[2] + 1, (WCell)qround(w16[9], M));
-                atomicAdd(W + G.cell[2] + 2, (WCell)qround(w16[10], M));
+                atomicAdd(W + G.cell[1] + 1, wq(w16[5], M));
+                atomicAdd(W + G.cell[1] + 2, wq(w16[6], M));
+                atomicAdd(W + G.cell[2] + 1, wq(w16[9], M));
+                atomicAdd(W + G.cell[2] + 2, wq(w16[10], M));
             }
         } else if (kind >= 2) {
             CorrW cw;
@@ -2081,7 +2097,7 @@ constexpr int kP2Cells = 4;
 
 template <bool QU, int CS, int P2 = kP2Cells, bool FC = false>
 __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, const FpRings& R,
-                                          const unsigned char* rtab, WCell* W, Acc4 acc) {
+                                          const unsigned char* rtab, WCell* W, double M, Acc4 acc) {
     constexpr int kP2 = P2;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -2098,8 +2114,9 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
         for (int u = 0; u < kP2; ++u) {
             const int j = base + u * kFpThreads + lane;
             const bool in = j < total;
-            wsum[u] = in ? (double)W[j] : 0.0;
-            w1sum[u] = (FC && in) ? (double)W1[j] : 0.0;
+            // (a cell of 0 quanta converts to exactly 0)
+            wsum[u] = in ? wval(W[j], M) : 0.0;
+            w1sum[u] = (FC && in) ? wval(W1[j], M) : 0.0;
             rr[u] = in ? (int)rtab[j] : 0;
         }
         // (mode 4: a cell is live when either grid holds weight)
@@ -2294,7 +2311,7 @@ __device__ __forceinline__ void fp_step(const TodArgs& an, bool have_next, int64
     else
         acc = fp_phase1<MODE, QU, CS, CLAIM, DUMP>(an, fp_job(buf), fr, R, buf, fp_W());
     __syncthreads();
-    acc = fp_phase2<QU, CS, P2, MODE == 4>(an, fr, R, fp_rtab(buf), fp_W(), acc);
+    acc = fp_phase2<QU, CS, P2, MODE == 4>(an, fr, R, fp_rtab(buf), fp_W(), uniform(fp_job(buf)->M), acc);
     // the pixels outside the layout, in pixel order (one thread)
     if (threadIdx.x == 0 && *fp_side_count() != 0)
         acc = fp_side_flush<QU, CS, MODE == 4>(fp_globals(an), fr, acc);
