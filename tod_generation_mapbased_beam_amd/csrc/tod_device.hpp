@@ -90,9 +90,10 @@ __device__ __forceinline__ bool finite_sample(const TodArgs& a, int64_t b) {
 // Weight-grid deposits are rounded to multiples of a quantum q = 2^(E - 52),
 // 2^E >= 64 sum_s |beam_vals[s]|, by (x + M) - M with M = 1.5 2^E (x + M lies
 // in [2^E, 2^(E+1)), whose spacing is q).  Every partial sum of a cell is then
-// a multiple of q below 2^53 q in magnitude, so binary64 adds are exact and
-// the cell's sum does not depend on the order of the atomic adds.  q is 2^-46
-// of the beam's total or finer: far below the binary32 weights' precision.
+// a multiple of q below 2^53 q in magnitude: the grid keeps it as an exact
+// integer count of quanta (WCell below), so the cell's sum does not depend on
+// the order of the atomic adds.  q is 2^-46 of the beam's total or finer: far
+// below the binary32 weights' precision.
 __device__ __forceinline__ double grid_magic(double sum_abs_beam) {
     if (!(sum_abs_beam > 0.0) || !(sum_abs_beam < 1e300)) return 0.0;  // no rounding
     int e = 0;
