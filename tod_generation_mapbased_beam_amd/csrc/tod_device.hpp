@@ -113,6 +113,15 @@ __device__ __forceinline__ double qround(double x, double M) { return (x + M) - 
 typedef unsigned long long WCell;
 __device__ __forceinline__ WCell wbits(double x) { return __builtin_bit_cast(WCell, x); }
 __device__ __forceinline__ WCell wq(double x, double M) { return wbits(x + M) - wbits(M); }
+// -bits(M) for a uniform M, opaque to the compiler: x + neg_bits(M) then stays
+// an add (one v_lshl_add_u64 with a scalar operand) instead of being folded
+// back into a 64-bit subtraction (a subtract-with-borrow pair and a move).
+// The empty asm emits nothing.
+__device__ __forceinline__ WCell neg_bits(double M) {
+    WCell n = 0ull - wbits(M);
+    asm("" : "+s"(n));
+    return n;
+}
 __device__ __forceinline__ double wval(WCell n, double M) { return __builtin_bit_cast(double, wbits(M) + n) - M; }
 
 struct Sample {
@@ -1283,11 +1292,11 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     // patterns of ua, va, ub, vb and M, which share one binade
     WCell* ca = W + ia;
     WCell* cb = W + ib;
-    const WCell bM = wbits(M), bva = wbits(va), bvb = wbits(vb);
+    const WCell nM = neg_bits(M), bva = wbits(va), bvb = wbits(vb);
     atomicAdd(ca, wbits(ua) - bva);
-    atomicAdd(ca + 1, bva - bM);
+    atomicAdd(ca + 1, bva + nM);
     atomicAdd(cb, wbits(ub) - bvb);
-    atomicAdd(cb + 1, bvb - bM);
+    atomicAdd(cb + 1, bvb + nM);
     if constexpr (FC) {
         // the correction grid: each deposit times (1 - g) g, grid-rounded
         const double uc = (double)fmaf(-gfr, gfr, gfr);
