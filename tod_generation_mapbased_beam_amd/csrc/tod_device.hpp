@@ -1301,10 +1301,11 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
         // the correction grid: each deposit times (1 - g) g, grid-rounded
         const double uc = (double)fmaf(-gfr, gfr, gfr);
         WCell* W1 = fp_W1();
-        atomicAdd(W1 + ia, wq(d0 * uc, M));
-        atomicAdd(W1 + ia + 1, wq(d1 * uc, M));
-        atomicAdd(W1 + ib, wq(d2 * uc, M));
-        atomicAdd(W1 + ib + 1, wq(d3 * uc, M));
+        // (d uc + M by one fma: the product rounded onto the grid once)
+        atomicAdd(W1 + ia, wbits(fma(d0, uc, M)) + nM);
+        atomicAdd(W1 + ia + 1, wbits(fma(d1, uc, M)) + nM);
+        atomicAdd(W1 + ib, wbits(fma(d2, uc, M)) + nM);
+        atomicAdd(W1 + ib + 1, wbits(fma(d3, uc, M)) + nM);
     }
     if constexpr (DUMP) {
         dd.pix[4 * di] = fp_cell_pixel(R, la, ia);
