@@ -2103,7 +2103,10 @@ constexpr float kSpinF32St2 = 3e-4f;
 // sample.  Cell j of the grid lies on footprint ring rtab[j]; a wave takes
 // 64 consecutive cells at a time, kP2Cells such chunks per round, so that
 // each lane has kP2Cells independent weight reads and gathers in flight.
-// Returns the per-thread sums (outside-grid sums included).
+// Returns the per-thread sums (outside-grid sums included).  Every slot of
+// the packed map is accumulated, padding included (its values are 0, and
+// the flush writes the first nslots sums only): a per-slot test would cost
+// each cell two selects per slot.
 constexpr int kP2Cells = 4;
 
 template <bool QU, int CS, int P2 = kP2Cells, bool FC = false>
@@ -2207,20 +2210,17 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
                     acc.v[1] += fma(uv, c2d, -(qv * s2d));
 #pragma unroll
                     for (int k2 = 2; k2 < CS; ++k2)
-                        if (k2 < a.nslots)
-                            acc.v[k2] += fma(w1, (double)slot(m1, k2), wsum_u * (double)slot(mv_u, k2));
+                        acc.v[k2] += fma(w1, (double)slot(m1, k2), wsum_u * (double)slot(mv_u, k2));
                 } else {
                     const double qv = (double)mv_u.x, uv = (double)mv_u.y;
                     acc.v[0] = fma(wsum_u, fma(qv, c2d, uv * s2d), acc.v[0]);
                     acc.v[1] = fma(wsum_u, fma(uv, c2d, -(qv * s2d)), acc.v[1]);
 #pragma unroll
-                    for (int k2 = 2; k2 < CS; ++k2)
-                        if (k2 < a.nslots) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
+                    for (int k2 = 2; k2 < CS; ++k2) acc.v[k2] = fma(wsum_u, (double)slot(mv_u, k2), acc.v[k2]);
                 }
             } else {
 #pragma unroll
                 for (int k2 = 0; k2 < CS; ++k2) {
-                    if (k2 >= a.nslots) continue;
                     if constexpr (FC)
                         acc.v[k2] += fma(w1sum[u], (double)slot(mv1[u], k2), wsum_u * (double)slot(mv_u, k2));
                     else
