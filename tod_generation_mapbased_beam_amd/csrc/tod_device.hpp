@@ -2236,17 +2236,48 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
 // halves the values per lane at each step (reduce-scatter: 7 exchanges of a
 // double instead of 4 x 6), leaving slot 2 (lane >= 32) + ((lane >> 4) & 1)
 // summed over each group of 16 lanes.
+// v of the lane that DPP control CTRL selects (every lane has a source for
+// the quad permutes and mirrors used here), both halves of the double.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// x + the partner lane's x across the halves of the wave for lanes 0..31
+// (partner l + 32), y + partner's y for lanes 32..63, by v_permlane32_swap:
+// it exchanges the upper half of one register with the lower half of the
+// other, and the two results add up to the pair sum whichever comes first.
+__device__ __forceinline__ double pair_sum32(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
+// The same across rows of 16 lanes (v_permlane16_swap): x + partner's x in
+// even rows (partner l + 16), y + partner's y in odd rows.
+__device__ __forceinline__ double pair_sum16(double x, double y) {
+    const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+    return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+
 __device__ __forceinline__ void fp_wave_reduce4(const Acc4& acc, double* red) {
     const int lane = threadIdx.x & 63;
     const bool hi = lane >= 32;
-    const double k0 = hi ? acc.v[2] : acc.v[0], k1 = hi ? acc.v[3] : acc.v[1];
-    const double s0 = hi ? acc.v[0] : acc.v[2], s1 = hi ? acc.v[1] : acc.v[3];
-    const double a0 = k0 + __shfl_xor(s0, 32, 64);
-    const double a1 = k1 + __shfl_xor(s1, 32, 64);
     const bool b4 = (lane >> 4) & 1;
-    double v = (b4 ? a1 : a0) + __shfl_xor(b4 ? a0 : a1, 16, 64);
-#pragma unroll
-    for (int off = 8; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    // lanes 0..31: slots 0 and 1 summed over l, l + 32; lanes 32..63: slots 2 and 3
+    const double a0 = pair_sum32(acc.v[0], acc.v[2]);
+    const double a1 = pair_sum32(acc.v[1], acc.v[3]);
+    // even rows: a0 summed over two rows, odd rows: a1
+    double v = pair_sum16(a0, a1);
+    // the sum over each row of 16 lanes by DPP, without LDS round trips:
+    // pairs, quads (quad_perm), half-rows (row_half_mirror), rows (row_mirror)
+    v += dpp_f64<0xb1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f64<0x4e>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
     if ((lane & 15) == 0) red[(hi ? 2 : 0) + (b4 ? 1 : 0)] = v;
 }
 
