@@ -1390,16 +1390,18 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
         const int r = la + k;
         // slot coordinate of phi n / 2 pi: the ring's (phi - phi0)/dphi
         // plus 1/2 on rings whose first pixel is shifted by half a slot
-        const double half = R.phi0(r) > 0.0 ? 0.5 : 0.0;
-        const double base = R.base(r) + half;
-        const double tc = fma(dl, R.inv_dphi(r), base);
+        const double b0 = R.base(r);
+        const double tc = fma(dl, R.inv_dphi(r), R.phi0(r) > 0.0 ? b0 + 0.5 : b0);
         const int j = (int)tc;
         const double f = tc - (double)j;  // <= 0 for tc < 0: rejected below
         ok &= (j + 1 < R.width(r)) & (f > kCertSlot) & (f < 1.0 - kCertSlot);
         const double ss = sth * R.sth(r), zz = z * R.z(r), dphi = R.dphi(r);
-        // phi_w - phi_c = dl + dphi (base - half - (j + q)) in window-relative slots
-        const double g0 = fma(dphi, base - half - (double)j, dl);
-        const double g1 = fma(dphi, base - half - (double)(j + 1), dl);
+        // phi_w - phi_c = dl + dphi (b0 - (j + q)) in window-relative slots
+        // (rounding here moves the distances by ulps, far inside the
+        // certification margins)
+        const double bj = b0 - (double)j;
+        const double g0 = fma(dphi, bj, dl);
+        const double g1 = fma(dphi, bj - 1.0, dl);
         const bool far0 = fabs(g1) < fabs(g0);
         const double gw = far0 ? g1 : g0, gl = far0 ? g0 : g1;
         ok &= fabs(gl) <= 0.05;
