@@ -83,8 +83,9 @@ int todg_gather_accum_fused(const float *vec_orig, const float *axes, const floa
  * outside a sample's footprint are summed in a fixed order), except for a
  * sample whose laid-out footprint misses more than 32 of its neighbour
  * pixels and, in bilinear mode, a sample whose footprint exceeds the grid
- * (4608 cells or 128 rings: a beam over ~3.7 deg in radius at nside 2048):
- * those agree to rounding. */
+ * (4608 cells or 128 rings: a beam over ~3.7 deg in radius at nside 2048) or
+ * a beam whose sum of |beam_vals| is zero or not finite (it runs without a
+ * weight grid): those agree to rounding. */
 
 /* Nearest-pixel beam-convolution TOD.  Replaces _gather_accum_nearest_jit
  * (tod_nearest.py:21-38, same argument order). */
