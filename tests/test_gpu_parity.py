@@ -274,6 +274,20 @@ def test_empty_and_single_inputs(K, mode, variant):
         assert err <= TOL
 
 
+@pytest.mark.parametrize("mode", ["bilinear", "nearest"])
+def test_non_finite_beam_value_propagates(K, mode):
+    """A NaN beam value makes every sample's TOD NaN, as the reference's sums
+    do (tod_bilinear.py:233-477 adds beam_val * value for every pixel); such a
+    beam has no weight grid and runs on the direct path."""
+    c = small_config()
+    B = 200
+    params = tuple(p[:B] for p in c["params"])
+    bv = c["bv"].copy()
+    bv[7] = np.nan
+    g = _gpu_tod(K, mode, c["vec"], params, c["nside"], c["map"], bv, B, c["S"], 1, 2)
+    assert np.all(np.isnan(g))
+
+
 def test_zero_beam_and_constant_map(K):
     rng = np.random.default_rng(41)
     nside, B, S = 32, 16, 40
