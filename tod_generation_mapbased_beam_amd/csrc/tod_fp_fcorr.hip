@@ -17,29 +17,6 @@ hipError_t launch_m4(const TodArgs& a, bool qu, int CS, hipStream_t st) {
     return launch_mode<4>(a, qu, CS, 0, st);
 }
 
-// Value of ring r2 at longitude phi by linear interpolation between the two
-// ring pixels whose centres bracket phi.  The callers' longitudes lie within
-// a few slots of [0, 2 pi) (a pixel's own longitude plus at most 3 of its
-// ring's slots, which are at most 1.75 slots of a ring 3 rings away), so one
-// conditional wrap replaces the 64-bit modulo; anything else takes the
-// general remainder.
-__device__ __forceinline__ double ring_lerp(const RingEntry* __restrict__ rt, const float* __restrict__ m,
-                                            int r2, double phi) {
-    const RingEntry& e = rt[r2];
-    const double t = (phi - e.phi0) * e.inv_dphi;
-    const double jf = floor(t);
-    const double f = t - jf;
-    const int n = e.n;
-    int j = (int)jf;
-    if (j < 0)
-        j += n;
-    else if (j >= n)
-        j -= n;
-    if ((unsigned)j >= (unsigned)n) j = (int)(((int64_t)jf % n + n) % n);
-    const int j1 = j + 1 == n ? 0 : j + 1;
-    return (1.0 - f) * (double)m[e.first + j] + f * (double)m[e.first + j1];
-}
-
 // Ring (1 .. 4 nside - 1) of RING pixel p.
 __device__ __forceinline__ int pixel_ring(int nside, int64_t p, int64_t npix) {
     const int64_t ncap = 2 * (int64_t)nside * (nside - 1);
@@ -57,11 +34,33 @@ __device__ __forceinline__ int pixel_ring(int nside, int64_t p, int64_t npix) {
     return 4 * nside - (int)((1 + s) >> 1);
 }
 
+// Value at slot coordinate t of a ring (n pixels from mb): linear
+// interpolation between the two pixels whose centres bracket t.  The
+// callers' coordinates lie within a few slots of [0, n) (a pixel's own
+// longitude plus at most 3 of its ring's slots, which are at most 1.75
+// slots of a ring 3 rings away), so one conditional wrap replaces the
+// 64-bit modulo; anything else takes the general remainder.
+__device__ __forceinline__ double ring_lerp_t(const float* __restrict__ mb, int n, double t) {
+    const double jf = floor(t);
+    const double f = t - jf;
+    int j = (int)jf;
+    if (j < 0)
+        j += n;
+    else if (j >= n)
+        j -= n;
+    if ((unsigned)j >= (unsigned)n) j = (int)(((int64_t)jf % n + n) % n);
+    const int j1 = j + 1 == n ? 0 : j + 1;
+    return (1.0 - f) * (double)mb[j] + f * (double)mb[j1];
+}
+
 // The 16 stencil features of pixel p on ring r (longitude phi): F_0 = m(p)
 // and, for (dr, ds) in {0..3}^2 minus (0, 0) in row-major order (feature
 // j = 4 dr + ds), the mean of the ring-interpolated values at rings r -+ dr,
 // longitudes phi -+ ds dphi_r, minus m(p) (features with dr > 0 are 0
-// within 3 rings of a pole).
+// within 3 rings of a pole).  On ring r itself those longitudes are pixel
+// centres, read directly; on each neighbour ring the slot coordinate of
+// phi is computed once and advances by dphi_r / dphi_ring per step.  Sums
+// in the order (a- + a+) + b- + b+ over rings a = r - dr, b = r + dr.
 __device__ __forceinline__ void fcorr_features(const RingEntry* __restrict__ rt, int nside,
                                                const float* __restrict__ m, int64_t p, int r,
                                                double phi, double f[16]) {
@@ -69,33 +68,51 @@ __device__ __forceinline__ void fcorr_features(const RingEntry* __restrict__ rt,
     const double mp = (double)m[p];
     const bool ok = r >= 4 && r <= 4 * nside - 4;
     f[0] = mp;
+    {
+        const int n = e.n;
+        const int i = (int)(p - e.first);
+        const float* __restrict__ mb = m + e.first;
 #pragma unroll
-    for (int dr = 0; dr < 4; ++dr) {
+        for (int ds = 1; ds < 4; ++ds) {
+            int il = i - ds, ih = i + ds;
+            if (il < 0) il += n;
+            if (ih >= n) ih -= n;
+            // (rings have >= 4 pixels, so one wrap suffices for ds <= 3)
+            f[ds] = 0.5 * ((double)mb[il] + (double)mb[ih]) - mp;
+        }
+    }
 #pragma unroll
-        for (int ds = 0; ds < 4; ++ds) {
-            if (dr == 0 && ds == 0) continue;
-            double v = 0.0;
-            if (dr == 0) {
-                v = 0.5 * (ring_lerp(rt, m, r, phi - ds * e.dphi) + ring_lerp(rt, m, r, phi + ds * e.dphi)) - mp;
-            } else if (ok) {
-                const int ra = r - dr, rb = r + dr;
-                if (ds == 0) {
-                    v = 0.5 * (ring_lerp(rt, m, ra, phi) + ring_lerp(rt, m, rb, phi)) - mp;
-                } else {
-                    const double pl = phi - ds * e.dphi, ph = phi + ds * e.dphi;
-                    v = 0.25 * (ring_lerp(rt, m, ra, pl) + ring_lerp(rt, m, ra, ph) + ring_lerp(rt, m, rb, pl) +
-                                ring_lerp(rt, m, rb, ph)) -
-                        mp;
+    for (int dr = 1; dr < 4; ++dr) {
+        if (ok) {
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const RingEntry& ea = rt[side == 0 ? r - dr : r + dr];
+                const double t0 = (phi - ea.phi0) * ea.inv_dphi;
+                const double step = e.dphi * ea.inv_dphi;
+                const float* __restrict__ mb = m + ea.first;
+                const int n = ea.n;
+                const double v0 = ring_lerp_t(mb, n, t0);
+                f[4 * dr] = side == 0 ? v0 : f[4 * dr] + v0;
+#pragma unroll
+                for (int ds = 1; ds < 4; ++ds) {
+                    const double vl = ring_lerp_t(mb, n, fma(-(double)ds, step, t0));
+                    const double vh = ring_lerp_t(mb, n, fma((double)ds, step, t0));
+                    f[4 * dr + ds] = side == 0 ? vl + vh : (f[4 * dr + ds] + vl) + vh;
                 }
             }
-            f[4 * dr + ds] = v;
+            f[4 * dr] = 0.5 * f[4 * dr] - mp;
+#pragma unroll
+            for (int ds = 1; ds < 4; ++ds) f[4 * dr + ds] = 0.25 * f[4 * dr + ds] - mp;
+        } else {
+#pragma unroll
+            for (int ds = 0; ds < 4; ++ds) f[4 * dr + ds] = 0.0;
         }
     }
 }
 
 // One thread per pixel: out0 = sum_j c[j] F_j, out1 = sum_j c[16 + j] F_j
 // (binary64 sums in feature order, stored binary32 at out[p * stride]).
-__global__ __launch_bounds__(256) void k_fcorr_channels(const RingEntry* __restrict__ rt, int nside,
+__global__ __launch_bounds__(256, 4) void k_fcorr_channels(const RingEntry* __restrict__ rt, int nside,
                                                          const float* __restrict__ m, FcorrCoef c,
                                                          float* __restrict__ out0, float* __restrict__ out1,
                                                          int64_t stride) {
@@ -178,7 +195,7 @@ __device__ __forceinline__ int smlp_perm(int kb, int g, int j) {
     return 16 * (2 * kb + (j >> 2)) + 4 * g + (j & 3);
 }
 
-__global__ __launch_bounds__(256) void k_smlp_channels(const RingEntry* __restrict__ rt, int nside,
+__global__ __launch_bounds__(256, 2) void k_smlp_channels(const RingEntry* __restrict__ rt, int nside,
                                                         const float* __restrict__ m,
                                                         const float* __restrict__ P, float* __restrict__ out0,
                                                         float* __restrict__ out1, int64_t stride) {
