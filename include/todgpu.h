@@ -44,6 +44,19 @@ int todg_version(void);
  * truncated to n-1 bytes); returns its length. */
 int todg_last_error(char *buf, size_t n);
 
+/* Index-check counters of the checked build (libtodgpu_checked.so, the same
+ * sources compiled with -DTODG_CHECKED; no counterpart in the reference,
+ * whose Numba kernels run with boundscheck off).  In that build every map
+ * gather, weight-grid deposit and ring-slot lookup tests its index, counts a
+ * violation by kind and still clamps, so nothing is accessed out of bounds.
+ * Synchronises the device, writes the counts of kinds 0 (map pixel outside
+ * [0, npix)), 1 (weight-grid cell outside the grid) and 2 (slot outside its
+ * ring or window), summed over the library's modules on the current device,
+ * into counts[0..n) (host memory; kinds beyond the third read 0), clears them
+ * when reset != 0 and returns the number of kinds (3).  The release library
+ * returns TODG_ERR_ARG. */
+int todg_check_counts(unsigned long long *counts, int n, int reset);
+
 /* Packed-map layout for a beam entry with C channels and Q/U positions c_q,
  * c_u (-1 = absent).  Writes the source channel of each packed slot (perm4,
  * -1 = zero) and the tod row each slot accumulates into (row4); returns the

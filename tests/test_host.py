@@ -37,6 +37,21 @@ def test_library_exports_every_declared_symbol():
     assert L.todg_version() == _lib.ABI_VERSION
 
 
+def test_checked_library_exports_the_same_symbols():
+    """libtodgpu_checked.so (the -DTODG_CHECKED build) binds the same C-ABI;
+    the release library refuses its counter query without touching a GPU."""
+    from tod_generation_mapbased_beam_amd import _lib
+
+    C = _lib._load(_lib.CHECKED_LIB_PATH)
+    for s in _declared_symbols():
+        assert hasattr(C, s), s
+    assert C.todg_version() == _lib.ABI_VERSION
+    buf = (ctypes.c_ulonglong * 3)()
+    assert _lib.lib().todg_check_counts(buf, 3, 1) == _lib.TODG_ERR_ARG
+    assert "TODG_CHECKED" in _lib.last_error()
+    assert _lib.lib().todg_check_counts(None, 3, 1) == _lib.TODG_ERR_ARG
+
+
 @pytest.mark.parametrize("nside", [1, 4, 128, 2048])
 def test_ring_table_bit_identical_to_oracle(nside):
     from tod_generation_mapbased_beam_amd import _lib

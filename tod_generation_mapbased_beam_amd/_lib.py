@@ -8,6 +8,7 @@ There is no fallback: if the library is missing, every call raises.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import threading
@@ -16,7 +17,11 @@ import torch  # noqa: F401  (must be loaded before the HIP library)
 
 _PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG_DIR, "libtodgpu.so")
+# the same sources compiled with -DTODG_CHECKED: index checks on every map
+# gather, weight-grid deposit and ring-slot lookup (see checked())
+CHECKED_LIB_PATH = os.path.join(_PKG_DIR, "libtodgpu_checked.so")
 ABI_VERSION = 1
+CHECK_KINDS = ("map_pixel", "grid_cell", "ring_slot")
 
 TODG_OK = 0
 TODG_ERR_ARG = -1
@@ -24,6 +29,7 @@ TODG_ERR_HIP = -2
 
 _lock = threading.Lock()
 _lib = None
+_handles: dict = {}
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -33,6 +39,7 @@ _i32 = ctypes.c_int
 SIGNATURES = {
     "todg_version": (_i32, []),
     "todg_last_error": (_i32, [ctypes.c_char_p, ctypes.c_size_t]),
+    "todg_check_counts": (_i32, [_p, _i32, _i32]),
     "todg_map_slots": (_i32, [_i32, _i32, _i32, _p, _p]),
     "todg_pack_map": (_i32, [_p, _i32, _i64, _i32, _i32, _p, _p]),
     "todg_gather_accum_fused": (
@@ -116,35 +123,70 @@ class TodgError(RuntimeError):
         self.code = code
 
 
-def lib() -> ctypes.CDLL:
-    """Load (once) and return the library; raises if it is not built."""
-    global _lib
-    if _lib is not None:
-        return _lib
+def _load(path: str) -> ctypes.CDLL:
     with _lock:
-        if _lib is not None:
-            return _lib
-        if not os.path.exists(LIB_PATH):
+        if path in _handles:
+            return _handles[path]
+        if not os.path.exists(path):
             raise RuntimeError(
-                f"{LIB_PATH} is missing: build it with "
+                f"{path} is missing: build it with "
                 "`python -m tod_generation_mapbased_beam_amd.build` (or __graft_entry__.build())"
             )
-        handle = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+        handle = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)
             fn.restype = res
             fn.argtypes = args
         v = handle.todg_version()
         if v != ABI_VERSION:
-            raise RuntimeError(f"libtodgpu ABI {v} != expected {ABI_VERSION}")
-        _lib = handle
+            raise RuntimeError(f"{os.path.basename(path)} ABI {v} != expected {ABI_VERSION}")
+        _handles[path] = handle
+        return handle
+
+
+def lib() -> ctypes.CDLL:
+    """Load (once) and return the library; raises if it is not built."""
+    global _lib
+    if _lib is None:
+        _lib = _load(LIB_PATH)
     return _lib
 
 
-def last_error() -> str:
+@contextlib.contextmanager
+def checked():
+    """Within the block every call of the package goes to the checked build
+    (libtodgpu_checked.so); read its violation counters with
+    check_counts().  Not thread-safe: it swaps the library the whole process
+    uses.  Device state the library owns (ring tables, queues) is per
+    library, so it is built again on first use."""
+    global _lib
+    prev = lib()
+    _lib = _load(CHECKED_LIB_PATH)
+    try:
+        yield _lib
+    finally:
+        _lib = prev
+
+
+def check_counts(reset: bool = True) -> dict:
+    """Index-check violations counted by the checked build since the last
+    reset, by kind (CHECK_KINDS); synchronises the device."""
+    buf = (ctypes.c_ulonglong * len(CHECK_KINDS))()
+    h = _load(CHECKED_LIB_PATH)
+    rc = h.todg_check_counts(buf, len(CHECK_KINDS), 1 if reset else 0)
+    if rc < 0:
+        raise TodgError(rc, _last_error(h))
+    return {k: int(buf[i]) for i, k in enumerate(CHECK_KINDS)}
+
+
+def _last_error(h: ctypes.CDLL) -> str:
     buf = ctypes.create_string_buffer(1024)
-    lib().todg_last_error(buf, len(buf))
+    h.todg_last_error(buf, len(buf))
     return buf.value.decode(errors="replace")
+
+
+def last_error() -> str:
+    return _last_error(lib())
 
 
 def check(rc: int) -> int:

@@ -331,6 +331,7 @@ __global__ __launch_bounds__(256) void k_features(int nside, const RingEntry* __
     float m[16];
     double s1 = 0.0;
     for (int k = 0; k < 16; ++k) {
+        TODG_CHECK((uint64_t)p16[k] < 12ull * (uint64_t)nside * (uint64_t)nside, kMapPixel);
         m[k] = field[p16[k]];
         s1 += m[k];
     }

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "check.hpp"
+
 namespace todg {
 
 constexpr double kTwoPi = 2.0 * 3.141592653589793;  // numba_healpy.py:42
@@ -467,6 +469,7 @@ __device__ __forceinline__ void nearest_pix(int nside, const RingEntry* __restri
 // negatives) so such cases read a valid pixel, never fault.
 template <int CS>
 __device__ __forceinline__ float4 load_px(const float* __restrict__ map, int64_t p, int64_t npix) {
+    TODG_CHECK((uint64_t)p < (uint64_t)npix, kMapPixel);
     p = (uint64_t)p < (uint64_t)npix ? p : ((p < 0) ? 0 : npix - 1);
     if constexpr (CS == 4) {
         return *(reinterpret_cast<const float4*>(map) + p);
@@ -484,6 +487,7 @@ __device__ __forceinline__ float4 load_px(const float* __restrict__ map, int64_t
 // clamp above would cost its compare-and-select chain per pixel.
 template <int CS>
 __device__ __forceinline__ float4 load_px_u32(const float* __restrict__ map, uint32_t p, uint32_t pmax) {
+    TODG_CHECK(p <= pmax, kMapPixel);
     p = min(p, pmax);
     if constexpr (CS == 4) {
         return *(reinterpret_cast<const float4*>(map) + (uint64_t)p);

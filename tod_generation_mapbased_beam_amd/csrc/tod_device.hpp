@@ -736,6 +736,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
         const int lr = ring - ir_lo;
         const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
         if (li >= 0) {
+            TODG_CHECK(li < fr->total, kGridCell);
             atomicAdd(&W[li], wq(w, M));
         } else {
             const Acc4 d = fr->direct ? fp_direct<QU, CS>(a, sp, ring, pos, w)
@@ -756,6 +757,7 @@ __device__ __noinline__ Acc4 fp_exact_pixel(FpGlobals a, int buf, float u0, floa
             const int lr = o.ring[k] - ir_lo;
             const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, o.pos[k]) : -1;
             if (li >= 0) {
+                TODG_CHECK(li < fr->total && fr->total <= kFpCapFc, kGridCell);
                 atomicAdd(&W[li], wq(w, M));
                 atomicAdd(&W1[li], wq(w1, M));
             } else {
@@ -1173,6 +1175,7 @@ __device__ __forceinline__ bool fp_layout(const TodArgs& a, int64_t b, const Bea
         fr->cha = cha;
         fr->ir_lo = ir_lo;
         fr->nr = nr;
+        TODG_CHECK(laid >= 0 && laid <= cap, kGridCell);
         fr->total = laid;  // grid cells [0, laid) belong to rings
         fr->direct = (nr == 0 || unfit) ? 1 : 0;
     }
@@ -1292,6 +1295,8 @@ __device__ __forceinline__ bool fp_fast_bilinear(const double* __restrict__ m, i
     // patterns of ua, va, ub, vb and M, which share one binade
     WCell* ca = W + ia;
     WCell* cb = W + ib;
+    TODG_CHECK(ia >= 0 && ib >= 0 && ia + 1 < (FC ? kFpCapFc : kFpCap) && ib + 1 < (FC ? kFpCapFc : kFpCap),
+               kGridCell);
     const WCell nM = neg_bits(M), bva = wbits(va), bvb = wbits(vb);
     atomicAdd(ca, wbits(ua) - bva);
     atomicAdd(ca + 1, bva + nM);
@@ -1417,6 +1422,7 @@ __device__ __forceinline__ bool fp_fast_nearest(const double* __restrict__ m, in
     ok &= (best - other > kCertNearest) & ((w1 ? lead[1] : lead[0]) > kCertNearest);
     if (!ok) return false;
     const double w = qround(bv, M);
+    TODG_CHECK((unsigned)cell < (unsigned)kFpCap, kGridCell);
     atomicAdd(W + cell, wq(bv, M));
     if constexpr (DUMP) {
         dd.pix[4 * di] = fp_cell_pixel(R, la + (w1 ? 1 : 0), cell);
@@ -1920,6 +1926,7 @@ __device__ __noinline__ Acc4 fp_corr_exact_deposit(FpGlobals a, int buf, float u
             const int lr = ring[r] - ir_lo;
             const int li = (lr >= 0 && lr < nr) ? fp_local(R, lr, pos) : -1;
             if (li >= 0) {
+                TODG_CHECK(li < fr->total, kGridCell);
                 atomicAdd(&W[li], wq(w, M));
             } else {
                 const Acc4 d = fr->direct ? fp_direct<QU, CS>(a, sp, ring[r], pos, w)
@@ -2069,6 +2076,10 @@ __device__ __forceinline__ Acc4 fp_phase1_corr(const TodArgs& a, FpJob* J, const
             w16[6] += wa * G.fa;
             w16[9] += wb - wb * G.fb;
             w16[10] += wb * G.fb;
+            TODG_CHECK(G.cell[0] >= 0 && G.cell[1] >= 0 && G.cell[2] >= 0 && G.cell[3] >= 0 &&
+                           G.cell[0] + 3 < kFpCap && G.cell[1] + 3 < kFpCap && G.cell[2] + 3 < kFpCap &&
+                           G.cell[3] + 3 < kFpCap,
+                       kGridCell);
             if (apply) {
 #pragma unroll
                 for (int r = 0; r < 4; ++r)
@@ -2150,6 +2161,7 @@ __device__ __forceinline__ Acc4 fp_phase2(const TodArgs& a, const FpFrame* fr, c
                 const int n = R.n(r);
                 if (pos >= n) pos -= n;
                 kk[u] = k;
+                TODG_CHECK((unsigned)k < (unsigned)R.width(r) && (unsigned)pos < (unsigned)n, kRingSlot);
                 // cell j < total lies in ring r's window: 0 <= k < width <= n
                 // and 0 <= i0 < n, so pos is in [0, n) and the pixel in the
                 // ring (npix < 2^31); the clamp only backs that up

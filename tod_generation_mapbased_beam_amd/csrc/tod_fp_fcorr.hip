@@ -50,6 +50,7 @@ __device__ __forceinline__ double ring_lerp_t(const float* __restrict__ mb, int 
         j -= n;
     if ((unsigned)j >= (unsigned)n) j = (int)(((int64_t)jf % n + n) % n);
     const int j1 = j + 1 == n ? 0 : j + 1;
+    TODG_CHECK((unsigned)j < (unsigned)n, kRingSlot);
     return (1.0 - f) * (double)mb[j] + f * (double)mb[j1];
 }
 
@@ -78,6 +79,7 @@ __device__ __forceinline__ void fcorr_features(const RingEntry* __restrict__ rt,
             if (il < 0) il += n;
             if (ih >= n) ih -= n;
             // (rings have >= 4 pixels, so one wrap suffices for ds <= 3)
+            TODG_CHECK((unsigned)il < (unsigned)n && (unsigned)ih < (unsigned)n, kRingSlot);
             f[ds] = 0.5 * ((double)mb[il] + (double)mb[ih]) - mp;
         }
     }

@@ -234,6 +234,7 @@ __global__ __launch_bounds__(256) void k_interp_points(int mode, int nside,
     double sum = 0.0;
     if (mode == 1) {
         sum = bicubic_visit(nside, rt, zz, pp, [&](int ring, int pos, double w) {
+            TODG_CHECK((unsigned)pos < (unsigned)rt[ring].n, kRingSlot);
             acc += w * (double)map[rt[ring].first + pos];
         });
     }
@@ -246,6 +247,8 @@ __global__ __launch_bounds__(256) void k_interp_points(int mode, int nside,
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             int64_t p = rt[o.ring[k]].first + o.pos[k];
+            // (p == npix in the float-mod corner at the last ring)
+            TODG_CHECK(p < npix, kMapPixel);
             p = p < npix ? p : npix - 1;
             acc += o.w[k] * (double)map[p];
         }
@@ -541,6 +544,27 @@ int todg_last_error(char* buf, size_t n) {
     memcpy(buf, g_last_error.data(), k);
     buf[k] = 0;
     return (int)k;
+}
+
+int todg_check_counts(unsigned long long* counts, int n, int reset) {
+    if (!counts || n < 0) return fail(TODG_ERR_ARG, "todg_check_counts: counts must be a host array, n >= 0");
+#ifdef TODG_CHECKED
+    unsigned long long acc[todg_check::kKinds] = {};
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(TODG_ERR_HIP, std::string("todg_check_counts: ") + hipGetErrorString(e));
+    for (auto rd : todg_check::readers()) {
+        const int rc = rd(acc, reset != 0);
+        if (rc != 0)
+            return fail(TODG_ERR_HIP, std::string("todg_check_counts: ") +
+                                               hipGetErrorString((hipError_t)rc));
+    }
+    for (int k = 0; k < n; ++k) counts[k] = k < todg_check::kKinds ? acc[k] : 0ull;
+    return todg_check::kKinds;
+#else
+    (void)reset;
+    for (int k = 0; k < n; ++k) counts[k] = 0ull;
+    return fail(TODG_ERR_ARG, "todg_check_counts: library built without index checks (TODG_CHECKED)");
+#endif
 }
 
 int todg_map_slots(int C, int c_q, int c_u, int* perm4, int* row4) {
