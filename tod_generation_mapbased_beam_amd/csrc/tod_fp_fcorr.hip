@@ -197,45 +197,69 @@ __device__ __forceinline__ int smlp_perm(int kb, int g, int j) {
     return 16 * (2 * kb + (j >> 2)) + 4 * g + (j & 3);
 }
 
-__global__ __launch_bounds__(256, 2) void k_smlp_channels(const RingEntry* __restrict__ rt, int nside,
+// The weights and biases live in LDS as per-lane MFMA fragments, read at
+// each use, rather than in ~100 VGPRs per lane: three workgroups (12 waves)
+// then fit on a CU instead of two, which hides more of the feature gathers'
+// latency.
+constexpr int kSmlpW1 = 0;                                 // [ft][lane]
+constexpr int kSmlpW2 = kSmlpW1 + kSmlpHT * 64;            // [ft][kb][lane]
+constexpr int kSmlpW3 = kSmlpW2 + kSmlpHT * kSmlpHB * 64;  // [ot][kb][lane]
+constexpr int kSmlpWFrags = kSmlpW3 + kSmlpOT * kSmlpHB * 64;
+constexpr int kSmlpB1 = 0, kSmlpB2 = kSmlpHT * 4, kSmlpB3 = 2 * kSmlpHT * 4;  // [tile][g]
+constexpr int kSmlpBFrags = (2 * kSmlpHT + kSmlpOT) * 4;
+
+__global__ __launch_bounds__(256, 3) void k_smlp_channels(const RingEntry* __restrict__ rt, int nside,
                                                         const float* __restrict__ m,
                                                         const float* __restrict__ P, float* __restrict__ out0,
                                                         float* __restrict__ out1, int64_t stride) {
     __shared__ __attribute__((aligned(16))) unsigned char tiles[4 * kSmlpTile];
+    // weights as MFMA A operands, biases as accumulator rows 4g .. 4g + 3
+    __shared__ bf16x8 wf[kSmlpWFrags];
+    __shared__ f32x4 bfr[kSmlpBFrags];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, g = lane >> 4;
     unsigned char* tile = tiles + wave * kSmlpTile;
-    // weights as MFMA A operands, biases as accumulator rows 4g .. 4g + 3
-    bf16x8 w1[kSmlpHT], w2[kSmlpHT][kSmlpHB], w3[kSmlpOT][kSmlpHB];
-    f32x4 b1[kSmlpHT], b2[kSmlpHT], b3[kSmlpOT];
+    // wave w lays out weight fragments w, w + 4, ... (fragment = one tile
+    // and k-block of one layer, 16 bytes per lane)
+    for (int fi = wave; fi < kSmlpWFrags / 64; fi += 4) {
+        bf16x8 v;
+        if (fi < kSmlpW2 / 64) {
+            const int o = 16 * fi + r16;
 #pragma unroll
-    for (int ft = 0; ft < kSmlpHT; ++ft) {
-        const int o = 16 * ft + r16;
+            for (int j = 0; j < 8; ++j) {
+                const int k = 8 * g + j;
+                v[j] = (__bf16)(k < smlp::kIn ? P[smlp::kOffW1 + o * smlp::kIn + k] : 0.f);
+            }
+        } else if (fi < kSmlpW3 / 64) {
+            const int ft = (fi - kSmlpW2 / 64) / kSmlpHB, kb = (fi - kSmlpW2 / 64) % kSmlpHB;
+            const int o = 16 * ft + r16;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int k = 8 * g + j;
-            w1[ft][j] = (__bf16)(k < smlp::kIn ? P[smlp::kOffW1 + o * smlp::kIn + k] : 0.f);
-        }
-#pragma unroll
-        for (int kb = 0; kb < kSmlpHB; ++kb)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) w2[ft][kb][j] = (__bf16)P[smlp::kOffW2 + o * smlp::kH + smlp_perm(kb, g, j)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            b1[ft][q] = P[smlp::kOffB1 + 16 * ft + 4 * g + q];
-            b2[ft][q] = P[smlp::kOffB2 + 16 * ft + 4 * g + q];
-        }
-    }
-#pragma unroll
-    for (int ot = 0; ot < kSmlpOT; ++ot) {
-#pragma unroll
-        for (int kb = 0; kb < kSmlpHB; ++kb)
+            for (int j = 0; j < 8; ++j) v[j] = (__bf16)P[smlp::kOffW2 + o * smlp::kH + smlp_perm(kb, g, j)];
+        } else {
+            const int ot = (fi - kSmlpW3 / 64) / kSmlpHB, kb = (fi - kSmlpW3 / 64) % kSmlpHB;
 #pragma unroll
             for (int j = 0; j < 8; ++j)
-                w3[ot][kb][j] = (__bf16)P[smlp::kOffW3 + (16 * ot + r16) * smlp::kH + smlp_perm(kb, g, j)];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) b3[ot][q] = P[smlp::kOffB3 + 16 * ot + 4 * g + q];
+                v[j] = (__bf16)P[smlp::kOffW3 + (16 * ot + r16) * smlp::kH + smlp_perm(kb, g, j)];
+        }
+        wf[64 * fi + lane] = v;
     }
+    if (threadIdx.x < kSmlpBFrags) {
+        const int t = threadIdx.x / 4, gg = threadIdx.x % 4;
+        const int off = t < kSmlpHT ? smlp::kOffB1 + 16 * t
+                                    : (t < 2 * kSmlpHT ? smlp::kOffB2 + 16 * (t - kSmlpHT)
+                                                       : smlp::kOffB3 + 16 * (t - 2 * kSmlpHT));
+        f32x4 v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = P[off + 4 * gg + q];
+        bfr[threadIdx.x] = v;
+    }
+    __syncthreads();
+    auto w1 = [&](int ft) { return wf[kSmlpW1 + 64 * ft + lane]; };
+    auto w2 = [&](int ft, int kb) { return wf[kSmlpW2 + 64 * (ft * kSmlpHB + kb) + lane]; };
+    auto w3 = [&](int ot, int kb) { return wf[kSmlpW3 + 64 * (ot * kSmlpHB + kb) + lane]; };
+    auto b1 = [&](int ft) { return bfr[kSmlpB1 + 4 * ft + g]; };
+    auto b2 = [&](int ft) { return bfr[kSmlpB2 + 4 * ft + g]; };
+    auto b3 = [&](int ot) { return bfr[kSmlpB3 + 4 * ot + g]; };
     const int64_t npix = 12 * (int64_t)nside * nside;
     const int64_t nchunks = (npix + 63) / 64;
     const int64_t wstride = (int64_t)gridDim.x * 4;
@@ -274,15 +298,15 @@ __global__ __launch_bounds__(256, 2) void k_smlp_channels(const RingEntry* __res
             f32x4 h[kSmlpHT];
             bf16x8 a[kSmlpHB];
 #pragma unroll
-            for (int ft = 0; ft < kSmlpHT; ++ft) h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1[ft], xin, b1[ft], 0, 0, 0);
+            for (int ft = 0; ft < kSmlpHT; ++ft) h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1(ft), xin, b1(ft), 0, 0, 0);
 #pragma unroll
             for (int kb = 0; kb < kSmlpHB; ++kb) a[kb] = corr_act(h[2 * kb], h[2 * kb + 1]);
 #pragma unroll
             for (int ft = 0; ft < kSmlpHT; ++ft) {
-                h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[ft][0], a[0], b2[ft], 0, 0, 0);
+                h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2(ft, 0), a[0], b2(ft), 0, 0, 0);
 #pragma unroll
                 for (int kb = 1; kb < kSmlpHB; ++kb)
-                    h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2[ft][kb], a[kb], h[ft], 0, 0, 0);
+                    h[ft] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w2(ft, kb), a[kb], h[ft], 0, 0, 0);
             }
 #pragma unroll
             for (int kb = 0; kb < kSmlpHB; ++kb) a[kb] = corr_act(h[2 * kb], h[2 * kb + 1]);
@@ -291,10 +315,10 @@ __global__ __launch_bounds__(256, 2) void k_smlp_channels(const RingEntry* __res
             __builtin_amdgcn_wave_barrier();
 #pragma unroll
             for (int ot = 0; ot < kSmlpOT; ++ot) {
-                f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[ot][0], a[0], b3[ot], 0, 0, 0);
+                f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3(ot, 0), a[0], b3(ot), 0, 0, 0);
 #pragma unroll
                 for (int kb = 1; kb < kSmlpHB; ++kb)
-                    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3[ot][kb], a[kb], o, 0, 0, 0);
+                    o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w3(ot, kb), a[kb], o, 0, 0, 0);
                 // row 16t + r16, outputs 16 ot + 4g .. +3
                 *reinterpret_cast<f32x4*>(row + 4 * (16 * ot + 4 * g)) = o;
             }
