@@ -52,6 +52,32 @@ def test_checked_library_exports_the_same_symbols():
     assert _lib.lib().todg_check_counts(None, 3, 1) == _lib.TODG_ERR_ARG
 
 
+def test_checked_context_swaps_and_restores_the_library():
+    """Inside _lib.checked() every call goes to the checked build (here a
+    host-only entry point, which must agree with the release one); the
+    release library is back afterwards, also after an exception."""
+    from tod_generation_mapbased_beam_amd import _lib
+
+    release = _lib.lib()
+
+    def table(h, nside=16):
+        n = 4 * nside + 1
+        arrs = [np.zeros(n) for _ in range(4)] + [np.zeros(n, np.int64), np.zeros(n, np.int64)]
+        _lib.check(h.todg_ring_table(nside, *(a.ctypes.data for a in arrs)))
+        return arrs
+
+    with _lib.checked() as C:
+        assert _lib.lib() is C and C is not release
+        got = table(_lib.lib())
+    assert _lib.lib() is release
+    for a, b in zip(got, table(release)):
+        assert np.array_equal(a, b)
+    with pytest.raises(RuntimeError):
+        with _lib.checked():
+            raise RuntimeError("inside")
+    assert _lib.lib() is release
+
+
 @pytest.mark.parametrize("nside", [1, 4, 128, 2048])
 def test_ring_table_bit_identical_to_oracle(nside):
     from tod_generation_mapbased_beam_amd import _lib
