@@ -84,3 +84,39 @@ def test_days_match_oracle(tmp_path, batch_size, prep):
         ref[1:] = t2.astype(np.float32)
         scale = np.sqrt(np.mean(ref ** 2, axis=1, keepdims=True))
         assert np.max(np.abs(got - ref) / scale) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch_size", [1200, 333])
+def test_device_day_pipeline_chunks_change_nothing(tmp_path, monkeypatch, batch_size):
+    """The device path's pipelined chunks (PIPE_CHUNK samples: two device
+    buffers, uploads and downloads on a second stream) give the day bit for
+    bit whatever the chunk size, ragged last chunks and several batches
+    included, and equal the same day computed in one launch."""
+    import torch
+
+    from tod_generation_mapbased_beam_amd import kernels
+
+    rng = np.random.default_rng(8)
+    nside = 32
+    mp = rng.standard_normal((3, 12 * nside * nside)).astype(np.float32)
+    vec, bv, _ = inputs.gaussian_beam(64, 240.0, 0.99)
+    bd = {"I": driver.beam_entry(vec, bv, [0], mp), "QU": driver.beam_entry(vec, bv, [1, 2], mp)}
+    n = 1200
+    th, ph, ps = inputs.ring_scan(n, 10.0)
+    driver.write_scan_day(str(tmp_path), 0, th, ph, ps)
+    days = {}
+    for chunk in (1 << 19, 97, 256):
+        monkeypatch.setattr(driver, "PIPE_CHUNK", chunk)
+        days[chunk] = driver.tod_exact_gen_batched(bd, 0, mp, batch_size, folder_scan=str(tmp_path), log=None)
+        assert days[chunk].shape == (3, n) and days[chunk].dtype == np.float64
+    assert np.array_equal(days[97], days[1 << 19]) and np.array_equal(days[256], days[1 << 19])
+    # one launch per entry over the whole day, as the unpipelined path ran it
+    ctr = R.beam_centre(np.zeros((1, 1)), np.full((1, 1), math.pi / 2))
+    params = kernels.rotation_params(np.array(th), np.array(ph), np.array(ps), ctr)
+    ref = torch.zeros((3, n), dtype=torch.float64, device="cuda")
+    for data in bd.values():
+        for comp, vals in driver.beam_tod_batch(nside, mp, data, None, None, None, None,
+                                                params=params).items():
+            ref[comp] += vals
+    assert np.array_equal(days[1 << 19], ref.cpu().numpy())

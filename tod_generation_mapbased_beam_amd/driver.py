@@ -10,7 +10,8 @@ the whole run (it replaces the reference's POSIX shared-memory copy,
 sample_based_tod_generation_gridint.py:478-507), so a batch can be a whole
 day.  The rotation operands are computed on the GPU from the uploaded
 pointings (``prep="device"``, the default: ``todg_rotation_params``, within
-one float32 ulp of the numpy path) and the day's TOD is summed in HBM; with
+one float32 ulp of the numpy path), in chunks whose memmap reads, uploads
+and downloads overlap the previous chunk's kernels; with
 ``prep="host"`` they come from the numpy path, the host preparing the next
 batch while the GPU works on the current one.  A failing day is reported and
 the others go on (sample_based_tod_generation_gridint.py:319-333).
@@ -68,27 +69,92 @@ def _host_batch(theta_mmap, phi_mmap, psi_mmap, bs, be, ra0, dec0):
     return rot_vecs, phi_b, theta_b, psis_b
 
 
+# Samples per pipelined chunk of the device path (see _device_day).
+PIPE_CHUNK = 1 << 19
+
+
 def _device_day(beam_data, nside, mp, theta_mmap, phi_mmap, psi_mmap, n_samples, batch_size, ra0, dec0,
                 interp_mode):
+    """The device path of one day, pipelined over chunks of at most
+    PIPE_CHUNK samples within each batch: while the GPU computes chunk k on
+    the current stream, the host reads chunk k + 1 from the memmaps into
+    pinned memory and a second stream (default priority) uploads it, and
+    chunk k - 1's TOD goes back to pinned host memory on that stream.  Two
+    device buffers alternate; events order each buffer's reuse.  Every
+    sample's TOD is computed on its own, so the chunking changes no value."""
     import torch
 
     from . import kernels
-    from .device import device
+    from .device import device, to_dev
 
+    if n_samples == 0:
+        return np.zeros((3, 0))
     dev = device()
     ctr = beam_centre(ra0, dec0)
-    tod_day = torch.zeros((3, n_samples), dtype=torch.float64, device=dev)
-    for bs in range(0, n_samples, batch_size):
-        be = min(bs + batch_size, n_samples)
-        pts = [torch.from_numpy(np.array(m[bs:be], dtype=np.float64)).to(dev)  # (a writeable copy)
-               for m in (theta_mmap, phi_mmap, psi_mmap)]
-        params = kernels.rotation_params(*pts, ctr)
-        for data in beam_data.values():
+    # the beam arrays on the device once (a pageable upload inside the loop
+    # would wait for the compute stream to drain)
+    beams = [{**d, "vec_orig": to_dev(d["vec_orig"], torch.float32),
+              "beam_vals": to_dev(d["beam_vals"], torch.float32)} for d in beam_data.values()]
+    spans = [(cs, min(cs + PIPE_CHUNK, bs + batch_size, n_samples))
+             for bs in range(0, n_samples, batch_size)
+             for cs in range(bs, min(bs + batch_size, n_samples), PIPE_CHUNK)]
+    width = max(ce - cs for cs, ce in spans)
+    compute = torch.cuda.current_stream(dev)
+    copy = torch.cuda.Stream(dev)
+    out = torch.empty((3, n_samples), dtype=torch.float64, pin_memory=True)
+    stage = [torch.empty((3, width), dtype=torch.float64, pin_memory=True) for _ in range(2)]
+    pts = [torch.empty((3, width), dtype=torch.float64, device=dev) for _ in range(2)]
+    tods = [torch.empty((3, width), dtype=torch.float64, device=dev) for _ in range(2)]
+    uploaded, consumed, downloaded = [None, None], [None, None], [None, None]
+
+    def upload(k):
+        cs, ce = spans[k]
+        i, n = k % 2, ce - cs
+        if uploaded[i] is not None:
+            uploaded[i].synchronize()  # the staging buffer's last upload is done
+        host = stage[i].numpy()
+        for row, m in enumerate((theta_mmap, phi_mmap, psi_mmap)):
+            np.copyto(host[row, :n], m[cs:ce], casting="same_kind")
+        with torch.cuda.stream(copy):
+            if consumed[i] is not None:
+                copy.wait_event(consumed[i])  # chunk k - 2 has read pts[i]
+            for row in range(3):
+                pts[i][row, :n].copy_(stage[i][row, :n], non_blocking=True)
+            uploaded[i] = torch.cuda.Event()
+            uploaded[i].record(copy)
+
+    def run(k):
+        cs, ce = spans[k]
+        i, n = k % 2, ce - cs
+        compute.wait_event(uploaded[i])
+        if downloaded[i] is not None:
+            compute.wait_event(downloaded[i])  # chunk k - 2's TOD has left tods[i]
+        tod = tods[i][:, :n]
+        tod.zero_()
+        p = pts[i]
+        params = kernels.rotation_params(p[0, :n], p[1, :n], p[2, :n], ctr)
+        for data in beams:
             contrib = beam_tod_batch(nside, mp, data, None, None, None, None, interp_mode=interp_mode,
                                      params=params)
             for comp, vals in contrib.items():
-                tod_day[comp, bs:be] += vals
-    return tod_day.cpu().numpy()
+                tod[comp] += vals
+        consumed[i] = torch.cuda.Event()
+        consumed[i].record(compute)
+        with torch.cuda.stream(copy):
+            copy.wait_event(consumed[i])
+            for row in range(3):
+                out[row, cs:ce].copy_(tod[row], non_blocking=True)
+            downloaded[i] = torch.cuda.Event()
+            downloaded[i].record(copy)
+
+    upload(0)
+    for k in range(len(spans)):
+        if k + 1 < len(spans):
+            upload(k + 1)
+        run(k)
+    copy.synchronize()
+    compute.synchronize()
+    return out.numpy()
 
 
 def tod_exact_gen_batched(beam_data, day_index, mp, batch_size, process_name=None, folder_scan=".",
