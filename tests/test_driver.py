@@ -38,6 +38,25 @@ def test_missing_day_is_reported_and_others_go_on(tmp_path):
     assert os.listdir(out) == []
 
 
+def test_host_params_split_over_threads_changes_nothing(tmp_path):
+    """The host path's rotation operands computed in pieces by a thread pool
+    equal the serial numpy path's bit for bit, ragged pieces included."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    n = 3 * driver.HOST_SPLIT_MIN + 12345
+    th, ph, ps = inputs.ring_scan(n, 20.0)
+    driver.write_scan_day(str(tmp_path), 0, th, ph, ps)
+    mm = driver.open_scan_day(str(tmp_path), 0)
+    ra0, dec0 = np.zeros((1, 1)), np.full((1, 1), math.pi / 2)
+    serial = driver._host_params(*mm, 5, n, ra0, dec0)
+    want = R._rotation_params(*driver._host_batch(*mm, 5, n, ra0, dec0))
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        split = driver._host_params(*mm, 5, n, ra0, dec0, pool, 7)
+    for a, b, c in zip(serial, split, want):
+        assert a.dtype == b.dtype == c.dtype and a.shape == b.shape == c.shape
+        assert np.array_equal(a, b) and np.array_equal(a, c)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("prep", ["host", "device"])
 @pytest.mark.parametrize("batch_size", [10_000, 333])

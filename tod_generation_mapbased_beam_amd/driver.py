@@ -12,8 +12,8 @@ day.  The rotation operands are computed on the GPU from the uploaded
 pointings (``prep="device"``, the default: ``todg_rotation_params``, within
 one float32 ulp of the numpy path), in chunks whose memmap reads, uploads
 and downloads overlap the previous chunk's kernels; with
-``prep="host"`` they come from the numpy path, the host preparing the next
-batch while the GPU works on the current one.  A failing day is reported and
+``prep="host"`` they come from the numpy path, split over host threads, the
+host preparing the next batch while the GPU works on the current one.  A failing day is reported and
 the others go on (sample_based_tod_generation_gridint.py:319-333).
 """
 
@@ -26,7 +26,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 from .tod_core import beam_tod_batch
-from .tod_rotations import beam_centre, precompute_rotation_vector_batch
+from .tod_rotations import _rotation_params, beam_centre, precompute_rotation_vector_batch
 
 
 def open_scan_day(folder_scan: str, day_index: int):
@@ -67,6 +67,28 @@ def _host_batch(theta_mmap, phi_mmap, psi_mmap, bs, be, ra0, dec0):
     rot_vecs, betas = precompute_rotation_vector_batch(ra0, dec0, phi_b, theta_b)
     psis_b = -betas + psi_b
     return rot_vecs, phi_b, theta_b, psis_b
+
+
+# Samples below which a batch's host preparation is not split over threads.
+HOST_SPLIT_MIN = 1 << 16
+
+
+def _host_params(theta_mmap, phi_mmap, psi_mmap, bs, be, ra0, dec0, pool=None, parts=1):
+    """The six kernel operands of samples [bs, be) by the numpy path
+    (_host_batch, then _rotation_params), the range split into ``parts``
+    pieces computed by ``pool``'s threads (numpy's loops release the GIL).
+    Every operation is per sample, so the split changes no value."""
+
+    def one(a, b):
+        return _rotation_params(*_host_batch(theta_mmap, phi_mmap, psi_mmap, a, b, ra0, dec0))
+
+    n = be - bs
+    parts = max(1, min(int(parts), n // HOST_SPLIT_MIN))
+    if pool is None or parts == 1:
+        return one(bs, be)
+    edges = [bs + (n * i) // parts for i in range(parts + 1)]
+    pieces = list(pool.map(lambda i: one(edges[i], edges[i + 1]), range(parts)))
+    return tuple(np.concatenate([p[j] for p in pieces]) for j in range(6))
 
 
 # Samples per pipelined chunk of the device path (see _device_day).
@@ -184,21 +206,23 @@ def tod_exact_gen_batched(beam_data, day_index, mp, batch_size, process_name=Non
             log(prefix + f"Done - {n_samples} samples in {total:.2f}s ({total / n_batches:.3f}s/batch)")
         return tod_day
     tod_day = np.zeros((3, n_samples))
-    with ThreadPoolExecutor(max_workers=1) as pool:
-        nxt = pool.submit(_host_batch, theta_mmap, phi_mmap, psi_mmap, 0, min(batch_size, n_samples),
-                          ra0, dec0)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=1) as ahead, ThreadPoolExecutor(max_workers=workers) as pool:
+        nxt = ahead.submit(_host_params, theta_mmap, phi_mmap, psi_mmap, 0, min(batch_size, n_samples),
+                           ra0, dec0, pool, workers)
         for k in range(n_batches):
             bs = k * batch_size
             be = min(bs + batch_size, n_samples)
-            rot_vecs, phi_b, theta_b, psis_b = nxt.result()
+            params = nxt.result()
             if k + 1 < n_batches:  # the host prepares batch k+1 while the GPU runs batch k
                 nb = min(be + batch_size, n_samples)
-                nxt = pool.submit(_host_batch, theta_mmap, phi_mmap, psi_mmap, be, nb, ra0, dec0)
+                nxt = ahead.submit(_host_params, theta_mmap, phi_mmap, psi_mmap, be, nb, ra0, dec0, pool,
+                                   workers)
             for data in beam_data.values():
-                contrib = beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b,
-                                         interp_mode=interp_mode)
+                contrib = beam_tod_batch(nside, mp, data, None, None, None, None, interp_mode=interp_mode,
+                                         params=params)
                 for comp, vals in contrib.items():
-                    tod_day[comp, bs:be] += vals
+                    tod_day[comp, bs:be] += vals.cpu().numpy()
     if log:
         total = time.time() - t0
         log(prefix + f"Done - {n_samples} samples in {total:.2f}s ({total / n_batches:.3f}s/batch)")
