@@ -217,3 +217,20 @@ def test_autotune_cache_roundtrip(tmp_path, monkeypatch):
     autotune._store(k, {"variant": 8})
     assert autotune.tuned_variant(2048, 6000, "bilinear", 3, True) == 8  # same bucket
     assert autotune.tuned_variant(2048, 9000, "bilinear", 3, True) == 0  # other bucket
+
+
+def test_rotation_params_split_equals_serial():
+    """beam_tod_batch's numpy path computes _rotation_params in pieces on a
+    thread pool: bit for bit the serial result, ragged pieces included."""
+    from tod_generation_mapbased_beam_amd import inputs
+    from tod_generation_mapbased_beam_amd import tod_rotations as R
+
+    n = 3 * R.SPLIT_MIN + 4321
+    th, ph, ps = inputs.ring_scan(n, 20.0)
+    phb, thb, psb = (np.asarray(x, np.float32) for x in (ph, th, ps))
+    rv, beta = R.precompute_rotation_vector_batch(np.zeros((1, 1)), np.full((1, 1), np.pi / 2), phb, thb)
+    want = R._rotation_params(rv, phb, thb, -beta + psb)
+    for workers in (1, 3, 16):
+        got = R._rotation_params_split(rv, phb, thb, -beta + psb, workers=workers)
+        for a, b in zip(want, got):
+            assert a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a, b)

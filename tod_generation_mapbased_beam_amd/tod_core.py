@@ -13,7 +13,7 @@ import torch
 
 from . import autotune, kernels
 from .device import device, to_dev
-from .tod_rotations import _rotation_params
+from .tod_rotations import _rotation_params_split
 
 
 def _qu_slots(comp_indices):
@@ -67,7 +67,7 @@ def beam_tod_batch(nside, mp, data, rot_vecs, phi_b, theta_b, psis_b, interp_mod
         if on_device:
             params = rotation_params_torch(rot_vecs, phi_b, theta_b, psis_b)
         else:
-            params = _rotation_params(rot_vecs, phi_b, theta_b, psis_b)
+            params = _rotation_params_split(rot_vecs, phi_b, theta_b, psis_b)
 
     if mp_stacked is None:
         mp_stacked = torch.stack([to_dev(mp[c], torch.float32) for c in comp_indices])

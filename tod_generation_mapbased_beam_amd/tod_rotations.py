@@ -99,6 +99,36 @@ def _rotation_params(rot_vecs, phi_b, theta_b, psis_b):
     return axes, cos_a, sin_a, ax_pts, np.cos(ps), np.sin(ps)
 
 
+# Samples below which _rotation_params_split does not split over threads.
+SPLIT_MIN = 1 << 16
+_pool = None
+
+
+def _rotation_params_split(rot_vecs, phi_b, theta_b, psis_b, workers=None):
+    """_rotation_params over pieces of the batch computed by a shared pool
+    of up to 16 host threads (numpy's loops release the GIL).  Every
+    operation is per sample, so the result equals _rotation_params' bit for
+    bit."""
+    global _pool
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    rot_vecs = np.asarray(rot_vecs)
+    n = int(rot_vecs.shape[0])
+    w = max(1, min(16, os.cpu_count() or 1)) if workers is None else int(workers)
+    parts = max(1, min(w, n // SPLIT_MIN))
+    if parts == 1:
+        return _rotation_params(rot_vecs, phi_b, theta_b, psis_b)
+    if _pool is None:
+        _pool = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+    ph, th, ps = (np.asarray(x) for x in (phi_b, theta_b, psis_b))
+    edges = [(n * i) // parts for i in range(parts + 1)]
+    pieces = list(_pool.map(lambda i: _rotation_params(rot_vecs[edges[i]:edges[i + 1]], ph[edges[i]:edges[i + 1]],
+                                                        th[edges[i]:edges[i + 1]], ps[edges[i]:edges[i + 1]]),
+                            range(parts)))
+    return tuple(np.concatenate([p[j] for p in pieces]) for j in range(6))
+
+
 def rotation_params_for_pointing(theta, phi, psi, ra=None, dec=None, center_idx=None):
     """Convenience: boresight (theta, phi, psi) -> the six kernel operands,
     following the driver's casts (sample_based_tod_generation_gridint.py:
