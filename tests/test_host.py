@@ -234,3 +234,22 @@ def test_rotation_params_split_equals_serial():
         got = R._rotation_params_split(rv, phb, thb, -beta + psb, workers=workers)
         for a, b in zip(want, got):
             assert a.dtype == b.dtype and a.shape == b.shape and np.array_equal(a, b)
+
+
+def test_rotation_params_pool_survives_fork():
+    """A process forked after the shared pool exists makes its own pool
+    (the inherited one has no threads) and gets the same operands."""
+    from tod_generation_mapbased_beam_amd import inputs
+    from tod_generation_mapbased_beam_amd import tod_rotations as R
+
+    n = 2 * R.SPLIT_MIN + 11
+    th, ph, ps = inputs.ring_scan(n, 20.0)
+    phb, thb, psb = (np.asarray(x, np.float32) for x in (ph, th, ps))
+    rv, beta = R.precompute_rotation_vector_batch(np.zeros((1, 1)), np.full((1, 1), np.pi / 2), phb, thb)
+    want = R._rotation_params_split(rv, phb, thb, -beta + psb)
+    pid = os.fork()
+    if pid == 0:  # child: exit status 0 iff the operands agree
+        got = R._rotation_params_split(rv, phb, thb, -beta + psb)
+        os._exit(0 if all(np.array_equal(a, b) for a, b in zip(want, got)) else 1)
+    _, status = os.waitpid(pid, 0)
+    assert os.WIFEXITED(status) and os.WEXITSTATUS(status) == 0

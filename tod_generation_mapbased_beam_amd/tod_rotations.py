@@ -102,14 +102,16 @@ def _rotation_params(rot_vecs, phi_b, theta_b, psis_b):
 # Samples below which _rotation_params_split does not split over threads.
 SPLIT_MIN = 1 << 16
 _pool = None
+_pool_pid = None
 
 
 def _rotation_params_split(rot_vecs, phi_b, theta_b, psis_b, workers=None):
     """_rotation_params over pieces of the batch computed by a shared pool
     of up to 16 host threads (numpy's loops release the GIL).  Every
     operation is per sample, so the result equals _rotation_params' bit for
-    bit."""
-    global _pool
+    bit.  The pool belongs to the process that made it: a forked child
+    (whose copy has no threads) makes its own."""
+    global _pool, _pool_pid
     import os
     from concurrent.futures import ThreadPoolExecutor
 
@@ -119,8 +121,9 @@ def _rotation_params_split(rot_vecs, phi_b, theta_b, psis_b, workers=None):
     parts = max(1, min(w, n // SPLIT_MIN))
     if parts == 1:
         return _rotation_params(rot_vecs, phi_b, theta_b, psis_b)
-    if _pool is None:
+    if _pool is None or _pool_pid != os.getpid():
         _pool = ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1)))
+        _pool_pid = os.getpid()
     ph, th, ps = (np.asarray(x) for x in (phi_b, theta_b, psis_b))
     edges = [(n * i) // parts for i in range(parts + 1)]
     pieces = list(_pool.map(lambda i: _rotation_params(rot_vecs[edges[i]:edges[i + 1]], ph[edges[i]:edges[i + 1]],
